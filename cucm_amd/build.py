@@ -1,0 +1,66 @@
+"""Builds the in-tree native libraries of cucm_amd.
+
+  libcucm.so          HIP kernels (gfx950) + host engine + C ABI (include/cucm.h)
+  libcucm_datagen.so  seeded synthetic input generators (host, OpenMP)
+
+Run:  python -m cucm_amd.build   (or __graft_entry__.build()).
+"""
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("CUCM_ARCH", "gfx950")
+
+LIB = os.path.join(HERE, "libcucm.so")
+DATAGEN = os.path.join(HERE, "libcucm_datagen.so")
+
+HIP_SOURCES = ["cucm_kernels.hip", "cucm_engine.cpp", "cucm_host_model.cpp", "cucm_serve.cpp"]
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
+    "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+    # packed FP32 instructions stay off (no float math on this path anyway)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if p.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd = [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        jobs.append((cmd, obj))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(jobs) + 1) as ex:
+        futs = [ex.submit(_run, cmd) for cmd, _ in jobs]
+        dg = ex.submit(_run, ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c11",
+                              "-o", DATAGEN + ".tmp", os.path.join(CSRC, "datagen.c"), "-lm"])
+        for f in futs:
+            out = f.result()
+            if verbose and out.strip():
+                print(out)
+        dg.result()
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", LIB + ".tmp"] + [o for _, o in jobs])
+    os.replace(LIB + ".tmp", LIB)
+    os.replace(DATAGEN + ".tmp", DATAGEN)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print("built", LIB, DATAGEN)

@@ -1,0 +1,286 @@
+"""Host-side mirror of CuCM's compress/decompress interface on the HIP codec.
+
+Reference interface (host C++): compress_file(ifstream, ofstream, n, &out_n,
+chunksize, unitlength, prelearn_size=0, batch_num=100)
+(cucm_compress_v2.cu:1151-1154); decompress(ifstream, n, ofstream, out_n,
+maxassumed_length=3) (cucm_decompress_v2.cu:1185); compress_batch /
+decompressbatch (cucm_compress_v2.cu:990-992, cucm_decompress_v2.cu:898-903).
+Every call goes through libcucm.so's C ABI; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from ._native import (FLAG_NO_TAIL_PAD, CucmError, Job, Params, Result, Stats, check, lib)
+
+MiB = 1 << 20
+
+
+def make_params(chunk_size=MiB, unit_bits=128, batch_chunks=100, prelearn_size=0, table_bits=25,
+                spec_bits=4, tail_pad=True):
+    p = Params()
+    lib().cucm_default_params(ctypes.byref(p))
+    p.chunk_size = chunk_size
+    p.unit_bits = unit_bits
+    p.batch_chunks = batch_chunks
+    p.prelearn_size = prelearn_size
+    p.table_bits = table_bits
+    p.spec_bits = spec_bits
+    p.flags = 0 if tail_pad else FLAG_NO_TAIL_PAD
+    return p
+
+
+def _buf(data):
+    """(address, length, keepalive) of a bytes-like or numpy object."""
+    if isinstance(data, np.ndarray):
+        a = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+        return a.ctypes.data, a.size, a
+    b = bytes(data)
+    cb = ctypes.create_string_buffer(b, len(b) or 1)
+    return ctypes.addressof(cb), len(b), cb
+
+
+def compress_bound(n, params):
+    return lib().cucm_compress_bound(n, ctypes.byref(params))
+
+
+def peek_header(blob):
+    addr, n, keep = _buf(blob)
+    orig = ctypes.c_uint64(0)
+    p = Params()
+    check(lib().cucm_peek_header(addr, n, ctypes.byref(orig), ctypes.byref(p)), "peek_header")
+    return orig.value, p
+
+
+class DeviceBuffer:
+    """Device memory on a context's GPU (cucm_ctx_malloc)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        check(lib().cucm_ctx_malloc(ctx.handle, ctypes.byref(self.ptr), max(self.nbytes, 1)), "malloc")
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def upload(self, data, offset=0):
+        addr, n, keep = _buf(data)
+        check(lib().cucm_ctx_memcpy_h2d(self.ctx.handle, self.addr + offset, addr, n), "h2d")
+
+    def download(self, n=None, offset=0) -> bytes:
+        n = self.nbytes - offset if n is None else n
+        out = np.empty(n, dtype=np.uint8)
+        check(lib().cucm_ctx_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.addr + offset, n), "d2h")
+        return out.tobytes()
+
+    def free(self):
+        if self.ptr.value:
+            lib().cucm_ctx_free(self.ctx.handle, self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A device context: one GPU, its stream, model slots and arenas."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.handle = ctypes.c_void_p()
+        check(lib().cucm_ctx_create(ctypes.byref(self.handle), device, hbm_budget), "ctx_create")
+
+    def close(self):
+        if self.handle and self.handle.value:
+            lib().cucm_ctx_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_max_waves(self, waves):
+        check(lib().cucm_ctx_set_max_waves(self.handle, waves))
+
+    def stats(self) -> dict:
+        st = Stats()
+        check(lib().cucm_ctx_last_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def synchronize(self):
+        check(lib().cucm_ctx_synchronize(self.handle))
+
+    def alloc(self, nbytes) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    # ---- host buffers -------------------------------------------------
+    def compress(self, data, params=None, **kw) -> bytes:
+        return self.compress_many([data], params, **kw)[0]
+
+    def decompress(self, blob, spec_bits=4) -> bytes:
+        return self.decompress_many([blob], spec_bits)[0]
+
+    def compress_many(self, datas, params=None, **kw):
+        n = len(datas)
+        if params is None:
+            params = make_params(**kw)
+        plist = params if isinstance(params, (list, tuple)) else [params] * n
+        bufs = [_buf(d) for d in datas]
+        ins = (ctypes.c_void_p * n)(*[b[0] for b in bufs])
+        lens = (ctypes.c_uint64 * n)(*[b[1] for b in bufs])
+        ps = (Params * n)(*plist)
+        caps = [int(compress_bound(b[1], p)) for b, p in zip(bufs, plist)]
+        # realistic output fits 1.5x; the library reports E_SMALL otherwise
+        caps = [min(c, b[1] + b[1] // 2 + 4096 * (1 + b[1] // max(p.chunk_size, 1)))
+                for c, b, p in zip(caps, bufs, plist)]
+        outs = [np.empty(max(c, 1), dtype=np.uint8) for c in caps]
+        outp = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
+        capa = (ctypes.c_uint64 * n)(*caps)
+        olen = (ctypes.c_uint64 * n)()
+        st = (ctypes.c_int32 * n)()
+        check(lib().cucm_compress_many(self.handle, n, ins, lens, ps, outp, capa, olen, st), "compress_many")
+        res = []
+        for i in range(n):
+            if st[i] == -3:  # retry this one with the strict bound
+                full = np.empty(int(compress_bound(bufs[i][1], plist[i])), dtype=np.uint8)
+                one = self._compress_one(bufs[i], plist[i], full)
+                res.append(one)
+                continue
+            check(st[i], "compress[%d]" % i)
+            res.append(outs[i][: olen[i]].tobytes())
+        return res
+
+    def _compress_one(self, b, p, out):
+        olen = ctypes.c_uint64(0)
+        check(lib().cucm_ctx_compress(self.handle, b[0], b[1], ctypes.byref(p), out.ctypes.data,
+                                      out.size, ctypes.byref(olen)), "compress")
+        return out[: olen.value].tobytes()
+
+    def decompress_many(self, blobs, spec_bits=4):
+        n = len(blobs)
+        bufs = [_buf(b) for b in blobs]
+        ins = (ctypes.c_void_p * n)(*[b[0] for b in bufs])
+        lens = (ctypes.c_uint64 * n)(*[b[1] for b in bufs])
+        sizes = [int.from_bytes(bytes(b)[:4], "big") if len(b) >= 4 else 0 for b in blobs]
+        outs = [np.empty(max(s, 1), dtype=np.uint8) for s in sizes]
+        outp = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
+        capa = (ctypes.c_uint64 * n)(*sizes)
+        olen = (ctypes.c_uint64 * n)()
+        st = (ctypes.c_int32 * n)()
+        check(lib().cucm_decompress_many(self.handle, n, ins, lens, outp, capa, olen, st, spec_bits),
+              "decompress_many")
+        res = []
+        for i in range(n):
+            check(st[i], "decompress[%d]" % i)
+            res.append(outs[i][: olen[i]].tobytes())
+        return res
+
+    # ---- batch records (compress_batch / decompressbatch) --------------
+    def compress_batch(self, data, chunk_size, unit_bits=128, table_bits=25) -> bytes:
+        addr, n, keep = _buf(data)
+        C = (n + chunk_size - 1) // chunk_size
+        cap = 4 * C + C * (32 * chunk_size + 1)
+        cap = min(cap, n + n // 2 + 4096 * (C + 1))
+        out = np.empty(cap, dtype=np.uint8)
+        olen = ctypes.c_uint64(0)
+        check(lib().cucm_compress_batch(self.handle, addr, n, chunk_size, unit_bits, table_bits,
+                                        out.ctypes.data, cap, ctypes.byref(olen)), "compress_batch")
+        return out[: olen.value].tobytes()
+
+    def decompress_batch(self, blob, n, chunk_size, unit_bits=128, table_bits=25, spec_bits=4) -> bytes:
+        addr, bl, keep = _buf(blob)
+        out = np.empty(n, dtype=np.uint8)
+        check(lib().cucm_decompress_batch(self.handle, addr, bl, out.ctypes.data, n, chunk_size,
+                                          unit_bits, table_bits, spec_bits), "decompress_batch")
+        return out.tobytes()
+
+    # ---- device-resident -------------------------------------------------
+    def compress_many_device(self, ins, lens, params, outs, caps, plain=False):
+        """ins/outs: device addresses; returns (out_len list, status list)."""
+        n = len(ins)
+        plist = params if isinstance(params, (list, tuple)) else [params] * n
+        a = (ctypes.c_void_p * n)(*ins)
+        l = (ctypes.c_uint64 * n)(*lens)
+        ps = (Params * n)(*plist)
+        o = (ctypes.c_void_p * n)(*outs)
+        c = (ctypes.c_uint64 * n)(*caps)
+        olen = (ctypes.c_uint64 * n)()
+        st = (ctypes.c_int32 * n)()
+        if plain:
+            check(lib().cucm_compress_plain_device(self.handle, n, a, l, ps, o, c, olen), "compress_plain")
+        else:
+            check(lib().cucm_compress_many_device(self.handle, n, a, l, ps, o, c, olen, st), "compress_device")
+        return list(olen), list(st)
+
+    def decompress_many_device(self, ins, lens, outs, caps, spec_bits=4, plain=False):
+        n = len(ins)
+        a = (ctypes.c_void_p * n)(*ins)
+        l = (ctypes.c_uint64 * n)(*lens)
+        o = (ctypes.c_void_p * n)(*outs)
+        c = (ctypes.c_uint64 * n)(*caps)
+        olen = (ctypes.c_uint64 * n)()
+        st = (ctypes.c_int32 * n)()
+        if plain:
+            check(lib().cucm_decompress_plain_device(self.handle, n, a, l, o, c, olen, spec_bits),
+                  "decompress_plain")
+        else:
+            check(lib().cucm_decompress_many_device(self.handle, n, a, l, o, c, olen, st, spec_bits),
+                  "decompress_device")
+        return list(olen), list(st)
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context()
+    return _default
+
+
+def compress(data, chunk_size=MiB, unit_bits=128, batch_chunks=100, prelearn_size=0, table_bits=25,
+             tail_pad=True) -> bytes:
+    p = make_params(chunk_size, unit_bits, batch_chunks, prelearn_size, table_bits, 4, tail_pad)
+    return default_context().compress(data, p)
+
+
+def decompress(blob, spec_bits=4) -> bytes:
+    return default_context().decompress(blob, spec_bits)
+
+
+def compress_file(infile, outfile, chunksize=MiB, unitlength=128, prelearn_size=0, batch_num=100,
+                  table_bits=25):
+    """Mirrors compress_file (cucm_compress_v2.cu:1151); returns the container size."""
+    with open(infile, "rb") as f:
+        data = f.read()
+    blob = compress(data, chunksize, unitlength, batch_num, prelearn_size, table_bits)
+    with open(outfile, "wb") as f:
+        f.write(blob)
+    return len(blob)
+
+
+def decompress_file(infile, outfile, maxassumed_length=3):
+    """Mirrors decompress (cucm_decompress_v2.cu:1185).  The reference decides
+    maxassumed_length+1 bits per speculative round; here the round width is
+    rounded to 1, 2, 4 or 8 bits (it never changes the output)."""
+    width = maxassumed_length + 1
+    spec = max(b for b in (1, 2, 4, 8) if b <= max(width, 1))
+    with open(infile, "rb") as f:
+        blob = f.read()
+    data = decompress(blob, spec)
+    with open(outfile, "wb") as f:
+        f.write(data)
+    return len(data)

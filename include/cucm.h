@@ -1,0 +1,184 @@
+/*
+ * cucm.h -- C ABI of the MI355X-native CuCM chunked context-mixing codec.
+ *
+ * Drop-in boundary for CuCM's compress/decompress path.  The reference has
+ * no library interface, only host C++ functions inside two CUDA programs;
+ * each entry point below names the reference function whose behaviour it
+ * replaces.  Containers produced with table_bits == 25 are byte-identical to
+ * the reference's (cucm_compress_v2.cu:1151-1250, format comment
+ * :1420-1443) and decode with the reference decoder.
+ *
+ * Conventions
+ *   - The caller owns every buffer.  No entry point prints or exit()s.
+ *   - Return 0 (CUCM_OK) or a negative CUCM_E_* code.
+ *   - A cucm_ctx is bound to one device and is safe to use from one host
+ *     thread at a time; distinct contexts may be used concurrently.
+ *   - The one-shot functions (cucm_compress, cucm_decompress) use a
+ *     process-wide context on device 0, serialised by a mutex.
+ */
+#ifndef CUCM_H
+#define CUCM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUCM_OK 0
+#define CUCM_E_PARAM (-1)       /* invalid parameter */
+#define CUCM_E_CORRUPT (-2)     /* malformed container */
+#define CUCM_E_SMALL (-3)       /* output buffer too small */
+#define CUCM_E_NOMEM (-4)       /* host or device allocation failed */
+#define CUCM_E_BUDGET (-5)      /* job does not fit the context's HBM budget */
+#define CUCM_E_DEVICE (-6)      /* HIP runtime error */
+#define CUCM_E_UNSUPPORTED (-7) /* valid for the format, not for this build */
+#define CUCM_E_NOTREADY (-8)    /* cucm_poll: job still running */
+
+/* Encoder option: code the file's last chunk only up to its real end instead
+ * of padding it to the nominal chunk size with stale bits as the reference
+ * does (cucm_compress_v2.cu:850-851).  The decoder ignores padding, so both
+ * forms decode identically; only the padded form is byte-identical to the
+ * reference encoder. */
+#define CUCM_FLAG_NO_TAIL_PAD 0x1u
+
+typedef struct cucm_params {
+  uint32_t chunk_size;    /* S, bytes per chunk (reference -s, default 1 MiB) */
+  uint16_t unit_bits;     /* U, bits per frozen-model unit, multiple of 32,
+                             32..256 in this build (reference -u, default 128) */
+  uint16_t batch_chunks;  /* chunks per batch record (reference -b, default 100) */
+  uint32_t prelearn_size; /* P, leading bytes that train the initial model (-p) */
+  uint8_t table_bits;     /* log2 slots per order table, 8..25; 25 = reference */
+  uint8_t spec_bits;      /* decoder bits per speculative round: 1, 2, 4 or 8
+                             (0 = default 4); never changes the stream */
+  uint16_t reserved;
+  uint32_t flags;         /* CUCM_FLAG_* */
+} cucm_params;
+
+/* Reference defaults: S = 1 MiB, U = 128, batch 100, P = 0, table_bits 25. */
+void cucm_default_params(cucm_params *p);
+const char *cucm_strerror(int code);
+
+/* Largest container cucm_compress can produce for in_len bytes (strict:
+ * the carry-less coder can emit up to 4 bytes per coded bit). */
+uint64_t cucm_compress_bound(uint64_t in_len, const cucm_params *p);
+
+/* Reads the 21-byte header (cucm_decompress_v2.cu:1189-1203,
+ * main's 4-byte size read :1297-1301). */
+int cucm_peek_header(const uint8_t *in, uint64_t in_len, uint64_t *orig_len, cucm_params *out);
+
+/* ---- one-shot, host buffers ------------------------------------------ */
+/* Replaces compress_file (cucm_compress_v2.cu:1151-1154) on a byte buffer. */
+int cucm_compress(const uint8_t *in, uint64_t in_len, const cucm_params *p, uint8_t *out,
+                  uint64_t out_cap, uint64_t *out_len);
+/* Replaces decompress (cucm_decompress_v2.cu:1185) on a byte buffer. */
+int cucm_decompress(const uint8_t *in, uint64_t in_len, uint8_t *out, uint64_t out_cap,
+                    uint64_t *out_len);
+
+/* ---- contexts ---------------------------------------------------------- */
+typedef struct cucm_ctx cucm_ctx;
+/* hbm_budget_bytes == 0 means 90% of the device's free memory. */
+int cucm_ctx_create(cucm_ctx **ctx, int device, uint64_t hbm_budget_bytes);
+void cucm_ctx_destroy(cucm_ctx *ctx);
+int cucm_ctx_compress(cucm_ctx *ctx, const uint8_t *in, uint64_t in_len, const cucm_params *p,
+                      uint8_t *out, uint64_t out_cap, uint64_t *out_len);
+int cucm_ctx_decompress(cucm_ctx *ctx, const uint8_t *in, uint64_t in_len, uint8_t *out,
+                        uint64_t out_cap, uint64_t *out_len);
+
+/* ---- batch records (no file header) ---------------------------------- */
+/* Replaces compress_batch (cucm_compress_v2.cu:990-992): encodes n bytes as
+ * ceil(n/S) chunks into one blob = 4C big-endian end offsets + payloads,
+ * starting from a fresh model (P = 0). */
+int cucm_compress_batch(cucm_ctx *ctx, const uint8_t *in, uint64_t n, uint32_t chunk_size,
+                        uint32_t unit_bits, uint32_t table_bits, uint8_t *blob, uint64_t cap,
+                        uint64_t *blob_len);
+/* Replaces decompressbatch (cucm_decompress_v2.cu:898-903); unlike the
+ * reference it does not modify any caller array. */
+int cucm_decompress_batch(cucm_ctx *ctx, const uint8_t *blob, uint64_t blob_len, uint8_t *out,
+                          uint64_t out_len, uint32_t chunk_size, uint32_t unit_bits,
+                          uint32_t table_bits, uint32_t spec_bits);
+
+/* ---- many files in one persistent launch ------------------------------ */
+/* Host buffers.  out_len[i] receives each result; the call fails as a whole
+ * on parameter errors, per-file container errors are reported in status[i]
+ * (status may be NULL). */
+int cucm_compress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
+                       const uint64_t in_len[], const cucm_params params[], uint8_t *const out[],
+                       const uint64_t out_cap[], uint64_t out_len[], int32_t status[]);
+int cucm_decompress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
+                         const uint64_t in_len[], uint8_t *const out[], const uint64_t out_cap[],
+                         uint64_t out_len[], int32_t status[], uint32_t spec_bits);
+/* Device-resident buffers (device pointers on the context's device); data
+ * never crosses PCIe except headers and sizes. */
+int cucm_compress_many_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const d_in[],
+                              const uint64_t in_len[], const cucm_params params[],
+                              uint8_t *const d_out[], const uint64_t out_cap[],
+                              uint64_t out_len[], int32_t status[]);
+int cucm_decompress_many_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const d_in[],
+                                const uint64_t in_len[], uint8_t *const d_out[],
+                                const uint64_t out_cap[], uint64_t out_len[], int32_t status[],
+                                uint32_t spec_bits);
+/* "Plain" path: the same work issued one file per launch (the reference's
+ * structure), for comparison with the persistent batched launch. */
+int cucm_compress_plain_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const d_in[],
+                               const uint64_t in_len[], const cucm_params params[],
+                               uint8_t *const d_out[], const uint64_t out_cap[],
+                               uint64_t out_len[]);
+int cucm_decompress_plain_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const d_in[],
+                                 const uint64_t in_len[], uint8_t *const d_out[],
+                                 const uint64_t out_cap[], uint64_t out_len[],
+                                 uint32_t spec_bits);
+
+/* ---- device memory helpers (for callers without their own allocator) -- */
+int cucm_ctx_malloc(cucm_ctx *ctx, void **dptr, uint64_t bytes);
+int cucm_ctx_free(cucm_ctx *ctx, void *dptr);
+int cucm_ctx_memcpy_h2d(cucm_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int cucm_ctx_memcpy_d2h(cucm_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int cucm_ctx_synchronize(cucm_ctx *ctx);
+
+/* ---- statistics of the last call on a context ------------------------- */
+typedef struct cucm_stats {
+  double wall_ms;          /* host wall time of the call */
+  double codec_ms;         /* encode/decode kernel, HIP events on the ctx stream */
+  double assemble_ms;      /* container layout + payload copy kernels */
+  double h2d_ms, d2h_ms;   /* transfers (host-buffer entry points only) */
+  uint64_t tasks;          /* chunks processed */
+  uint64_t waves;          /* resident worker waves of the persistent launch */
+  uint64_t launches;       /* codec kernel launches */
+  uint64_t in_bytes, out_bytes;
+  uint64_t model_bytes;    /* HBM held by model slots */
+} cucm_stats;
+int cucm_ctx_last_stats(cucm_ctx *ctx, cucm_stats *st);
+
+/* Upper bound on concurrently resident worker waves (0 = automatic). */
+int cucm_ctx_set_max_waves(cucm_ctx *ctx, uint32_t waves);
+
+/* ---- serving: continuous batching ------------------------------------ */
+#define CUCM_OP_COMPRESS 0
+#define CUCM_OP_DECOMPRESS 1
+typedef struct cucm_job {
+  int32_t op;              /* CUCM_OP_* */
+  int32_t reserved;
+  const uint8_t *in;       /* host buffer */
+  uint64_t in_len;
+  cucm_params params;      /* compress only */
+  uint8_t *out;            /* host buffer */
+  uint64_t out_cap;
+} cucm_job;
+typedef struct cucm_result {
+  int32_t status;          /* CUCM_OK or CUCM_E_* */
+  int32_t reserved;
+  uint64_t out_len;
+  double queue_ms;         /* submit -> start on device */
+  double latency_ms;       /* submit -> result ready */
+} cucm_result;
+/* Starts the serving loop on ctx (max_inflight jobs resident at once). */
+int cucm_serve_start(cucm_ctx *ctx, uint32_t max_inflight);
+int cucm_serve_stop(cucm_ctx *ctx);
+int cucm_submit(cucm_ctx *ctx, const cucm_job *job, uint64_t *job_id);
+/* CUCM_E_NOTREADY while running; blocking waits if wait != 0. */
+int cucm_poll(cucm_ctx *ctx, uint64_t job_id, int wait, cucm_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
