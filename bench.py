@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X CuCM codec (one JSON line on stdout).
+
+Default workload (the metric's single-GPU configuration): a 256 MiB mixed
+synthetic stream (60% order-3 Markov text, 25% LZ copies, 15% random 4 KiB
+blocks; seed 2) cut into 1 MiB chunks, reference model with 2^22 slots per
+order table, U = 128, P = 0, 100 chunks per batch record.
+
+One step = compress the device-resident input into a device-resident
+container, then decompress it back (both through the C ABI, one persistent
+launch each).  `value` is input MB (1e6 B) per second of step time; the
+compress and decompress rates are reported beside it.  Outputs are checked
+after the timed region: the round trip must be lossless, and the payloads of
+a sample of chunks must equal the oracle's (the CPU restatement of the
+reference), which is also timed as the CPU baseline.
+
+Other workloads: --workload batch (10,000 lognormal files, batched vs
+plain launch), --workload serve (Poisson request mix, p50/p99).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MiB = 1 << 20
+METRIC = "compress/decompress MB/s on 1 GPU at identical ratio; batched vs plain; p99"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per input byte: 8 bits x 8 orders one-byte state
+# gathers + the same number of one-byte state write-backs, 1 B of input and
+# ~1/ratio B of output (order-0 and statep/mixer live in LDS).
+ALG_BYTES_PER_INPUT_BYTE = 128.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stream_workload(args):
+    import cucm_amd
+    from cucm_amd import datagen
+
+    n = args.mib * MiB
+    S, U, tb, batch = 1 * MiB, 128, 22, 100
+    t0 = time.time()
+    data = datagen.mixed(n, seed=2)
+    log("generated %d MiB mixed stream in %.1fs" % (args.mib, time.time() - t0))
+    ctx = cucm_amd.Context()
+    params = cucm_amd.make_params(S, U, batch, 0, tb)
+    C = (n + S - 1) // S
+    cap = n + n // 2 + 4096 * (C + 1)
+    d_in = ctx.alloc(n)
+    d_in.upload(data)
+    d_comp = ctx.alloc(cap)
+    d_out = ctx.alloc(n)
+    ctx.synchronize()
+
+    def compress():
+        lens, st = ctx.compress_many_device([d_in.addr], [n], params, [d_comp.addr], [cap])
+        if st[0] != 0:
+            raise RuntimeError("compress status %d" % st[0])
+        return lens[0], ctx.stats()
+
+    def decompress(clen):
+        lens, st = ctx.decompress_many_device([d_comp.addr], [clen], [d_out.addr], [n], args.spec)
+        if st[0] != 0 or lens[0] != n:
+            raise RuntimeError("decompress status %d len %d" % (st[0], lens[0]))
+        return ctx.stats()
+
+    for i in range(args.warmup):
+        clen, _ = compress()
+        decompress(clen)
+        log("warmup %d done" % i)
+    ctx.synchronize()
+    tc, td, kc, kd = [], [], [], []
+    t_all0 = time.perf_counter()
+    for i in range(args.steps):
+        a = time.perf_counter()
+        clen, sc = compress()
+        b = time.perf_counter()
+        sd = decompress(clen)
+        c = time.perf_counter()
+        tc.append(b - a)
+        td.append(c - b)
+        kc.append(sc["codec_ms"])
+        kd.append(sd["codec_ms"])
+        log("step %d: compress %.3fs (kernel %.1f ms)  decompress %.3fs (kernel %.1f ms)"
+            % (i, b - a, sc["codec_ms"], c - b, sd["codec_ms"]))
+    ctx.synchronize()
+    t_all = time.perf_counter() - t_all0
+    # checks outside the timed region
+    back = np.frombuffer(d_out.download(n), dtype=np.uint8)
+    lossless = bool(np.array_equal(back, data))
+    container = d_comp.download(clen)
+    ratio = n / clen
+    res = {
+        "compress_MBps": n / 1e6 / float(np.median(tc)),
+        "decompress_MBps": n / 1e6 / float(np.median(td)),
+        "compress_kernel_ms": float(np.median(kc)),
+        "decompress_kernel_ms": float(np.median(kd)),
+        "ratio": ratio,
+        "lossless": lossless,
+        "container_bytes": clen,
+        "waves": sc["waves"],
+        "model_bytes": sc["model_bytes"],
+    }
+    ms_per_step = 1000.0 * t_all / args.steps
+    value = n / 1e6 / (t_all / args.steps)
+    # dominant kernel: the slower of the two launches
+    dom = "decode" if res["decompress_kernel_ms"] >= res["compress_kernel_ms"] else "encode"
+    kms = max(res["compress_kernel_ms"], res["decompress_kernel_ms"])
+    achieved = ALG_BYTES_PER_INPUT_BYTE * n / (kms / 1000.0) / 1e9
+    roofline = {"bound": "hbm", "kernel": "codec_kernel<%s>" % dom, "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None}
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline(data, container, S, U, tb, args)
+    line = {
+        "metric": METRIC, "value": value, "unit": "MB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32 fixed-point",
+        "data": "synthetic (seeded generator, resident in HBM)",
+        "config": {"workload": "mixed stream %d MiB, 1 MiB chunks, 2^22 slots/order, U=128, P=0"
+                   % args.mib, "chunks": C, "chunk_size": S, "unit_bits": U, "table_bits": tb,
+                   "batch_chunks": batch, "spec_bits": args.spec},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    line.update(res)
+    line["oracle_sample_identical"] = parity
+    return line
+
+
+def chunk_payloads(container, nchunks_max):
+    """Yields (chunk index, payload) of the first chunks of a container."""
+    import struct
+    P = struct.unpack(">I", container[16:20])[0]
+    Pc = struct.unpack(">I", container[12:16])[0]
+    flag = container[20]
+    N = struct.unpack(">I", container[0:4])[0]
+    batch = struct.unpack(">H", container[4:6])[0]
+    S = struct.unpack(">I", container[8:12])[0]
+    pos = 21 + (Pc if flag else P)
+    C = (N - P + S - 1) // S
+    out = []
+    c0 = 0
+    while c0 < C and len(out) < nchunks_max:
+        Cb = min(batch, C - c0)
+        blen = struct.unpack("<I", container[pos:pos + 4])[0]
+        blob = container[pos + 4:pos + 4 + blen]
+        prev = 4 * Cb - 1
+        for c in range(Cb):
+            e = struct.unpack(">I", blob[4 * c:4 * c + 4])[0]
+            out.append((c0 + c, blob[prev + 1:e + 1]))
+            prev = e
+            if len(out) >= nchunks_max:
+                break
+        pos += 4 + blen
+        c0 += Cb
+    return out
+
+
+def cpu_baseline(data, container, S, U, tb, args):
+    """Times the oracle (a port of the reference model) on a bounded sample of
+    the same workload: `sample` whole chunks, one chunk per core (OpenMP)."""
+    from oracle import oracle as O
+
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    k = args.cpu_chunks or threads
+    sample = data[: k * S].tobytes()
+    model = O.Model(tb)
+    t0 = time.perf_counter()
+    blob = O.encode_batch(sample, S, U, model, tail_pad=True, threads=threads)
+    t1 = time.perf_counter()
+    back = O.decode_batch(blob, len(sample), S, U, model, threads=threads)
+    t2 = time.perf_counter()
+    ok = back == sample
+    # compare the oracle's payloads with the GPU container's first k chunks
+    import struct
+    gpu = dict(chunk_payloads(container, k))
+    prev = 4 * k - 1
+    same = True
+    for c in range(k):
+        e = struct.unpack(">I", blob[4 * c:4 * c + 4])[0]
+        if gpu.get(c) != blob[prev + 1:e + 1]:
+            same = False
+        prev = e
+    cpu = {"value": len(sample) / 1e6 / (t1 - t0), "unit": "MB/s (compress)",
+           "decompress_MBps": len(sample) / 1e6 / (t2 - t1), "cores": threads, "kind": "port",
+           "sample": "%d chunks x %d MiB of the same stream (first %d MiB), oracle encode_batch/"
+                     "decode_batch, OpenMP one chunk per core" % (k, S // MiB, k * S // MiB),
+           "lossless": ok}
+    return cpu, same
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="stream", choices=["stream", "batch", "serve"])
+    ap.add_argument("--mib", type=int, default=256)
+    ap.add_argument("--spec", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-chunks", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the codec shards by chunk with no exchange on the "
+              "hot loop, and this build runs on a single MI355X (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "stream":
+        line = stream_workload(args)
+    elif args.workload == "batch":
+        from cucm_amd.workloads import batch_workload
+        line = batch_workload(args)
+    else:
+        from cucm_amd.workloads import serve_workload
+        line = serve_workload(args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -23,6 +23,7 @@ EXPORTS = [
     "cucm_decompress_many_device", "cucm_compress_plain_device", "cucm_decompress_plain_device",
     "cucm_ctx_malloc", "cucm_ctx_free", "cucm_ctx_memcpy_h2d", "cucm_ctx_memcpy_d2h",
     "cucm_ctx_synchronize", "cucm_ctx_last_stats", "cucm_ctx_set_max_waves",
+    "cucm_ctx_set_phase_profile", "cucm_ctx_phase_cycles",
     "cucm_serve_start", "cucm_serve_stop", "cucm_submit", "cucm_poll",
 ]
 
@@ -131,6 +132,8 @@ def lib():
         L.cucm_ctx_synchronize.argtypes = [vp]
         L.cucm_ctx_last_stats.argtypes = [vp, P(Stats)]
         L.cucm_ctx_set_max_waves.argtypes = [vp, u32]
+        L.cucm_ctx_set_phase_profile.argtypes = [vp, ctypes.c_int]
+        L.cucm_ctx_phase_cycles.argtypes = [vp, P(u64)]
         L.cucm_serve_start.argtypes = [vp, u32]
         L.cucm_serve_stop.argtypes = [vp]
         L.cucm_submit.argtypes = [vp, P(Job), P(u64)]

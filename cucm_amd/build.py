@@ -20,7 +20,8 @@ ARCH = os.environ.get("CUCM_ARCH", "gfx950")
 LIB = os.path.join(HERE, "libcucm.so")
 DATAGEN = os.path.join(HERE, "libcucm_datagen.so")
 
-HIP_SOURCES = ["cucm_kernels.hip", "cucm_engine.cpp", "cucm_host_model.cpp", "cucm_serve.cpp"]
+HIP_SOURCES = ["cucm_kernels.hip", "cucm_engine.cpp", "cucm_host_model.cpp", "cucm_serve.cpp",
+               "gather_probe.hip"]
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,

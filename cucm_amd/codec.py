@@ -114,6 +114,17 @@ class Context:
     def set_max_waves(self, waves):
         check(lib().cucm_ctx_set_max_waves(self.handle, waves))
 
+    PHASES = ["init", "stage", "hash+gather", "tables+mix", "coder", "mixer+collide", "resolve",
+              "statep", "writeback", "dec_paths", "dec_mix+decode", "dec_records"]
+
+    def set_phase_profile(self, on=True):
+        check(lib().cucm_ctx_set_phase_profile(self.handle, 1 if on else 0))
+
+    def phase_cycles(self) -> dict:
+        arr = (ctypes.c_uint64 * 12)()
+        check(lib().cucm_ctx_phase_cycles(self.handle, arr))
+        return dict(zip(self.PHASES, list(arr)))
+
     def stats(self) -> dict:
         st = Stats()
         check(lib().cucm_ctx_last_stats(self.handle, ctypes.byref(st)))

@@ -233,6 +233,23 @@ int upload_model(cucm_ctx *c, const HostModel &m, InitModels &im) {
 
 }  // namespace
 
+// Arms the optional per-phase cycle counters for one codec launch.
+static int prof_arm(cucm_ctx *c, CodecArgs &a) {
+  a.prof = nullptr;
+  if (!c->profile) return CUCM_OK;
+  CU_TRY(c->prof.reserve(8 * kPhases));
+  HIP_TRY(hipMemsetAsync(c->prof.p, 0, 8 * kPhases, c->stream));
+  a.prof = (uint64_t *)c->prof.p;
+  return CUCM_OK;
+}
+
+static int prof_collect(cucm_ctx *c, const CodecArgs &a) {
+  if (!a.prof) return CUCM_OK;
+  HIP_TRY(hipMemcpyAsync(c->phase, a.prof, 8 * kPhases, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUCM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Encode
 // ---------------------------------------------------------------------------
@@ -346,11 +363,13 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
   a.result = (uint64_t *)c->results.p;
   a.tabs = (const DevTables *)c->tabs.p;
   a.init = (const DevInitModel *)c->initm.p;
+  CU_TRY(prof_arm(c, a));
   HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   HIP_TRY(launch_codec(false, a, Umax, sh.blocks, sh.wpb, c->stream));
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   st.launches += 1;
+  CU_TRY(prof_collect(c, a));
   std::vector<uint64_t> res(ntasks);
   HIP_TRY(hipMemcpyAsync(res.data(), c->results.p, 8ull * ntasks, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -615,11 +634,13 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   a.result = (uint64_t *)c->results.p;
   a.tabs = (const DevTables *)c->tabs.p;
   a.init = (const DevInitModel *)c->initm.p;
+  CU_TRY(prof_arm(c, a));
   HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   HIP_TRY(launch_codec(true, a, Umax, sh.blocks, sh.wpb, c->stream));
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   st.launches += 1;
+  CU_TRY(prof_collect(c, a));
   std::vector<int32_t> stv(pj.size());
   HIP_TRY(hipMemcpyAsync(stv.data(), c->status.p, 4 * pj.size(), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -747,6 +768,18 @@ void cucm_ctx_destroy(cucm_ctx *ctx) { delete ctx; }
 int cucm_ctx_set_max_waves(cucm_ctx *ctx, uint32_t waves) {
   if (!ctx) return CUCM_E_PARAM;
   ctx->max_waves = waves;
+  return CUCM_OK;
+}
+
+int cucm_ctx_set_phase_profile(cucm_ctx *ctx, int enable) {
+  if (!ctx) return CUCM_E_PARAM;
+  ctx->profile = enable != 0;
+  return CUCM_OK;
+}
+
+int cucm_ctx_phase_cycles(cucm_ctx *ctx, uint64_t cycles[CUCM_PHASES]) {
+  if (!ctx || !cycles) return CUCM_E_PARAM;
+  for (int i = 0; i < CUCM_PHASES; ++i) cycles[i] = ctx->phase[i];
   return CUCM_OK;
 }
 

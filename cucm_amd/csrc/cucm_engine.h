@@ -42,9 +42,11 @@ struct cucm_ctx {
   uint64_t budget = 0;
   uint32_t cus = 0, lds_cu = 0, lds_block = 0;
   uint32_t max_waves = 0;
+  bool profile = false;
+  uint64_t phase[cucm::kPhases] = {};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   DevBuf tabs, pool, staging, overflow, tasks, tasks2, results, results2, taskdst, asmjobs,
-      headers, joblen, counter, initm, ptrs, hdrs, parsejobs, status, arena_in, arena_out;
+      headers, joblen, counter, initm, ptrs, hdrs, parsejobs, status, arena_in, arena_out, prof;
   PinnedBuf pin;
   std::vector<cucm::Task> last_tasks;
   std::vector<uint64_t> last_results;

@@ -31,16 +31,15 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 constexpr uint16_t kNoState = 0x100;
 constexpr uint32_t kWindow = 256;  // decoder payload window in LDS
 
+// Pointers into LDS carry the LDS address space so every access is a ds_* op.
+#define LDS __attribute__((address_space(3)))
+#define GLB __attribute__((address_space(1)))
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Makes this wave's global stores visible to its own later global loads.
-__device__ __forceinline__ void global_sync() {
-  __threadfence();
-  __builtin_amdgcn_wave_barrier();
 }
 
 __host__ __device__ inline uint32_t table_size(uint32_t U) {
@@ -54,80 +53,133 @@ __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15
 // Per-wave LDS carve-out.  Laid out for the launch's largest unit (Umax)
 // and, for the decoder, the speculation width (P paths).
 struct WaveMem {
-  uint32_t *stp;    // [8][256] statep
-  int32_t *w;       // [80][8] mixer weights
-  uint32_t *tkey;   // [7][TS] distinct slots of the unit, orders 1..7
-  uint32_t *pslot;  // [P][8] decoder: slot (order 0: h0) per path
-  uint16_t *tval;   // [7][TS] running state of each distinct slot
-  uint16_t *ridx;   // [8][Umax] table index (order 0: h0) per unit bit
-  uint16_t *rprm;   // [Umax] pr << 1 | y per unit bit (persists: tail padding)
-  int16_t *psmp;    // [P][8] decoder: stretched prediction per path
-  uint16_t *ppr;    // [P] decoder: mixed probability per path
-  uint8_t *st0;     // [256] order-0 states by h0
-  uint8_t *rg;      // [8][Umax] unit-start state per unit bit
-  uint8_t *rcxt;    // [Umax] mixer context per unit bit
-  uint8_t *pg;      // [P][8] decoder: unit-start state per path
-  uint8_t *pcxt;    // [P] decoder: mixer context per path
-  uint8_t *ub;      // [8 + Umax/8] 8 history bytes + the unit's bytes
-  uint8_t *win;     // [kWindow] decoder payload window
-  uint8_t *chosen;  // [8] decoder: path taken by each bit of a round
+  LDS uint32_t *stp;    // [8][256] statep
+  LDS int32_t *w;       // [80][8] mixer weights
+  LDS uint32_t *tkey;   // [7][TS] distinct slots of the unit, orders 1..7
+  LDS uint32_t *pslot;  // [P][8] decoder: slot (order 0: h0) per path
+  LDS uint16_t *tval;   // [7][TS] running state of each distinct slot
+  LDS uint16_t *ridx;   // [8][Umax] table index (order 0: h0) per unit bit
+  LDS uint16_t *rprm;   // [Umax] pr << 1 | y per unit bit (persists: tail padding)
+  LDS int16_t *psmp;    // [P][8] decoder: stretched prediction per path
+  LDS uint16_t *ppr;    // [P] decoder: mixed probability per path
+  LDS uint8_t *st0;     // [256] order-0 states by h0
+  LDS uint8_t *rg;      // [8][Umax] unit-start state per unit bit
+  LDS uint8_t *rcxt;    // [Umax] mixer context per unit bit
+  LDS uint8_t *pg;      // [P][8] decoder: unit-start state per path
+  LDS uint8_t *pcxt;    // [P] decoder: mixer context per path
+  LDS uint8_t *ub;      // [8 + Umax/8] 8 history bytes + the unit's bytes
+  LDS uint8_t *win;     // [kWindow] decoder payload window
+  LDS uint8_t *chosen;  // [8] decoder: path taken by each bit of a round
 };
 
-__host__ __device__ inline uint32_t wave_lds_bytes(uint32_t Umax, uint32_t P, bool dec,
-                                                   WaveMem *m = nullptr, uint8_t *base = nullptr) {
+// Byte offsets of the per-wave LDS arrays (16-byte aligned each).
+struct WaveLayout {
+  uint32_t stp, w, tkey, pslot, tval, ridx, rprm, psmp, ppr, st0, rg, rcxt, pg, pcxt, ub, win,
+      chosen, total;
+};
+
+__host__ __device__ __forceinline__ WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec) {
   const uint32_t TS = table_size(Umax);
+  const uint32_t Pd = dec ? P : 0u;
+  WaveLayout l;
   uint32_t off = 0;
-  auto take = [&](uint32_t bytes) {
-    uint32_t at = off;
-    off = align16(off + bytes);
-    return at;
-  };
-  uint32_t o_stp = take(8 * 256 * 4);
-  uint32_t o_w = take(kMixCtx * kOrders * 4);
-  uint32_t o_tkey = take(7 * TS * 4);
-  uint32_t o_pslot = take(dec ? P * 8 * 4 : 0);
-  uint32_t o_tval = take(7 * TS * 2);
-  uint32_t o_ridx = take(8 * Umax * 2);
-  uint32_t o_rprm = take(Umax * 2);
-  uint32_t o_psmp = take(dec ? P * 8 * 2 : 0);
-  uint32_t o_ppr = take(dec ? P * 2 : 0);
-  uint32_t o_st0 = take(256);
-  uint32_t o_rg = take(8 * Umax);
-  uint32_t o_rcxt = take(Umax);
-  uint32_t o_pg = take(dec ? P * 8 : 0);
-  uint32_t o_pcxt = take(dec ? P : 0);
-  uint32_t o_ub = take(8 + Umax / 8 + 8);
-  uint32_t o_win = take(dec ? kWindow : 0);
-  uint32_t o_chosen = take(dec ? 8 : 0);
-  if (m) {
-    m->stp = (uint32_t *)(base + o_stp);
-    m->w = (int32_t *)(base + o_w);
-    m->tkey = (uint32_t *)(base + o_tkey);
-    m->pslot = (uint32_t *)(base + o_pslot);
-    m->tval = (uint16_t *)(base + o_tval);
-    m->ridx = (uint16_t *)(base + o_ridx);
-    m->rprm = (uint16_t *)(base + o_rprm);
-    m->psmp = (int16_t *)(base + o_psmp);
-    m->ppr = (uint16_t *)(base + o_ppr);
-    m->st0 = base + o_st0;
-    m->rg = base + o_rg;
-    m->rcxt = base + o_rcxt;
-    m->pg = base + o_pg;
-    m->pcxt = base + o_pcxt;
-    m->ub = base + o_ub;
-    m->win = base + o_win;
-    m->chosen = base + o_chosen;
-  }
-  return off;
+#define CUCM_TAKE(field, bytes) \
+  l.field = off;                \
+  off = align16(off + (bytes));
+  CUCM_TAKE(stp, 8 * 256 * 4)
+  CUCM_TAKE(w, kMixCtx * kOrders * 4)
+  CUCM_TAKE(tkey, 7 * TS * 4)
+  CUCM_TAKE(pslot, Pd * 8 * 4)
+  CUCM_TAKE(tval, 7 * TS * 2)
+  CUCM_TAKE(ridx, 8 * Umax * 2)
+  CUCM_TAKE(rprm, Umax * 2)
+  CUCM_TAKE(psmp, Pd * 8 * 2)
+  CUCM_TAKE(ppr, Pd * 2)
+  CUCM_TAKE(st0, 256)
+  CUCM_TAKE(rg, 8 * Umax)
+  CUCM_TAKE(rcxt, Umax)
+  CUCM_TAKE(pg, Pd * 8)
+  CUCM_TAKE(pcxt, Pd)
+  CUCM_TAKE(ub, 8 + Umax / 8 + 8)
+  CUCM_TAKE(win, dec ? kWindow : 0u)
+  CUCM_TAKE(chosen, dec ? 8u : 0u)
+#undef CUCM_TAKE
+  l.total = off;
+  return l;
 }
+
+__host__ __device__ inline uint32_t wave_lds_bytes(uint32_t Umax, uint32_t P, bool dec) {
+  return wave_layout(Umax, P, dec).total;
+}
+
+__device__ __forceinline__ WaveMem carve(LDS uint8_t *base, uint32_t Umax, uint32_t P, bool dec) {
+  const WaveLayout l = wave_layout(Umax, P, dec);
+  WaveMem m;
+  m.stp = (LDS uint32_t *)(base + l.stp);
+  m.w = (LDS int32_t *)(base + l.w);
+  m.tkey = (LDS uint32_t *)(base + l.tkey);
+  m.pslot = (LDS uint32_t *)(base + l.pslot);
+  m.tval = (LDS uint16_t *)(base + l.tval);
+  m.ridx = (LDS uint16_t *)(base + l.ridx);
+  m.rprm = (LDS uint16_t *)(base + l.rprm);
+  m.psmp = (LDS int16_t *)(base + l.psmp);
+  m.ppr = (LDS uint16_t *)(base + l.ppr);
+  m.st0 = base + l.st0;
+  m.rg = base + l.rg;
+  m.rcxt = base + l.rcxt;
+  m.pg = base + l.pg;
+  m.pcxt = base + l.pcxt;
+  m.ub = base + l.ub;
+  m.win = base + l.win;
+  m.chosen = base + l.chosen;
+  return m;
+}
+
+// Per-phase shader-clock totals of one wave (enabled when args.prof != null):
+//   0 model init   1 unit staging   2 hash + gather   3 slot tables + mix
+//   4 arithmetic coder   5 mixer deltas + collision scan   6 state resolution
+//   7 statep recurrences   8 write-back   9 decoder path predictions
+//   10 decoder mix + decode   11 decoder records
+struct PhaseClock {
+  bool on;
+  uint64_t last;
+  uint64_t acc[kPhases];
+  __device__ __forceinline__ explicit PhaseClock(bool enable) : on(enable), last(0) {
+#pragma unroll
+    for (int i = 0; i < kPhases; ++i) acc[i] = 0;
+    if (on) last = clock64();
+  }
+  __device__ __forceinline__ void mark(int ph) {
+    if (on) {
+      const uint64_t t = clock64();
+      acc[ph] += t - last;
+      last = t;
+    }
+  }
+  __device__ __forceinline__ void flush(uint64_t *out, uint32_t lane) {
+    if (on && lane == 0)
+#pragma unroll
+      for (int i = 0; i < kPhases; ++i) atomicAdd((unsigned long long *)&out[i], (unsigned long long)acc[i]);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Model primitives (device)
 // ---------------------------------------------------------------------------
 
+// Waits for this wave's outstanding vector memory operations (s_waitcnt
+// vmcnt(0)): a wave's state write-backs must land before its own next
+// gathers.  No cache maintenance is needed for a wave's own accesses.
+__device__ __forceinline__ void vm_drain() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0), expcnt/lgkmcnt untouched
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // Mixer::p (cucm_compress_v2.cu:636-644) plus the pr bump (:787-788).
-__device__ __forceinline__ uint32_t mix_predict(const int32_t *wrow, const int32_t smp[8],
-                                                const uint16_t *squash) {
+__device__ __forceinline__ uint32_t mix_predict(const LDS int32_t *wrow, const int32_t smp[8],
+                                                const LDS uint16_t *squash) {
   int64_t dot = 0;
 #pragma unroll
   for (int k = 0; k < 8; ++k) dot += (int64_t)wrow[k] * (int64_t)smp[k];
@@ -141,35 +193,65 @@ __device__ __forceinline__ uint32_t mix_predict(const int32_t *wrow, const int32
 // Carry-less coder interval split (cucm_compress_v2.cu:855-857).
 __device__ __forceinline__ uint32_t coder_mid(uint32_t x1, uint32_t x2, uint32_t pr) {
   const uint32_t rng = x2 - x1;
-  return x1 + __umul24(rng >> 12, pr) + (__umul24(rng & 0xfffu, pr) >> 12);
+  return x1 + (rng >> 12) * pr + (((rng & 0xfffu) * pr) >> 12);
 }
 
-// Inserts `slot` into the order's distinct-slot table; returns its index.
-__device__ __forceinline__ uint32_t table_insert(uint32_t *tkey, uint32_t TS, uint32_t slot) {
-  uint32_t h = slot & (TS - 1);
-  for (;;) {
-    const uint32_t old = atomicCAS(&tkey[h], kEmpty, slot);
-    if (old == kEmpty || old == slot) return h;
-    h = (h + 1) & (TS - 1);
+// StateMap adaptation rate dt[n] = 16384 / (2n + 3) (cucm_compress_v2.cu:702-704):
+// a reciprocal estimate corrected by one remainder step, exact for n < 1024.
+__device__ __forceinline__ uint32_t adapt_rate(uint32_t n) {
+  const uint32_t m = 2 * n + 3;
+  uint32_t q = (uint32_t)(16384.0f * __builtin_amdgcn_rcpf((float)m));
+  const int32_t rem = 16384 - (int32_t)(q * m);
+  q += (rem >= (int32_t)m) ? 1u : 0u;
+  q -= (rem < 0) ? 1u : 0u;
+  return q;
+}
+
+// Inserts the 7 slots of orders 1..7 into the unit's distinct-slot tables;
+// the first probe of every order is issued before any result is examined.
+__device__ __forceinline__ void insert_slots(LDS uint32_t *tkey, uint32_t TS, const uint32_t sl[8],
+                                             uint32_t idx[8]) {
+  uint32_t old[8];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    idx[k] = sl[k] & (TS - 1);
+    old[k] = atomicCAS((uint32_t *)&tkey[(k - 1) * TS + idx[k]], kEmpty, sl[k]);
   }
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    while (old[k] != kEmpty && old[k] != sl[k]) {
+      idx[k] = (idx[k] + 1) & (TS - 1);
+      old[k] = atomicCAS((uint32_t *)&tkey[(k - 1) * TS + idx[k]], kEmpty, sl[k]);
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t insert_slot(LDS uint32_t *tk, uint32_t TS, uint32_t slot) {
+  uint32_t h = slot & (TS - 1);
+  uint32_t old = atomicCAS((uint32_t *)&tk[h], kEmpty, slot);
+  while (old != kEmpty && old != slot) {
+    h = (h + 1) & (TS - 1);
+    old = atomicCAS((uint32_t *)&tk[h], kEmpty, slot);
+  }
+  return h;
 }
 
 // Zeroes (or copies the prelearned model into) the wave's model slot and
 // resets the LDS-resident parts of the model.
-__device__ void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M, uint8_t *slot,
+__device__ __forceinline__ void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M, GLB uint8_t *slot,
                            uint32_t Umax, uint32_t lane) {
   const uint64_t bytes = 7ull << tk.tb;
   const uint64_t n16 = bytes >> 4;
-  uint4 *dst = (uint4 *)slot;
+  GLB u32x4 *dst = (GLB u32x4 *)slot;
   if (tk.flags & kTaskInitModel) {
     const DevInitModel *im = a.init + tk.model;
-    const uint4 *src = (const uint4 *)im->states;
+    const GLB u32x4 *src = (const GLB u32x4 *)im->states;
     for (uint64_t i = lane; i < n16; i += 64) dst[i] = src[i];
     for (uint32_t i = lane; i < 8 * 256; i += 64) M.stp[i] = im->statep[i >> 8][i & 255];
     for (uint32_t i = lane; i < kMixCtx * kOrders; i += 64) M.w[i] = im->w[i];
     for (uint32_t i = lane; i < 256; i += 64) M.st0[i] = im->st0[i];
   } else {
-    const uint4 z = make_uint4(0, 0, 0, 0);
+    const u32x4 z = {0u, 0u, 0u, 0u};
     for (uint64_t i = lane; i < n16; i += 64) dst[i] = z;
     for (uint32_t i = lane; i < 8 * 256; i += 64) M.stp[i] = 1u << 31;
     for (uint32_t i = lane; i < kMixCtx * kOrders; i += 64) M.w[i] = 0;
@@ -182,85 +264,174 @@ __device__ void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M,
   }
   for (uint32_t i = lane; i < Umax; i += 64) M.rprm[i] = 0;
   for (uint32_t i = lane; i < 8 + Umax / 8 + 8; i += 64) M.ub[i] = 0;
-  global_sync();
+  vm_drain();
   wave_sync();
 }
 
 // Applies the deferred updates of one unit (cucm_compress_v2.cu:794-839,
-// cucm_decompress_v2.cu:870-896): mixer deltas from the recorded (pr, y,
-// cxt) and unit-start predictions, then every order's StateMap updates in
-// bit order, then the distinct slots' final states back to HBM.
-__device__ void update_unit(const WaveMem &M, const DevTables *T, uint8_t *slot, uint32_t tb,
-                            uint32_t Umax, uint32_t nvalid, uint32_t lane) {
+// cucm_decompress_v2.cu:870-896), for unit bits r < nvalid (a multiple of 8):
+//   1. mixer deltas from the recorded (pr, y, cxt) and unit-start predictions
+//      (LDS integer atomics: the sum is order independent);
+//   2. state resolution: the state each StateMap update sees, s = state of
+//      the slot after all earlier bits of the unit.  Bits of one byte never
+//      share an order-0 slot (distinct partial bytes), so a byte's 8 bits x
+//      8 orders go in one step over all 64 lanes; a byte where two bits of an
+//      order hash to one slot (a hash collision) is stepped bit by bit;
+//   3. per order, the statep recurrence in bit order (8 lanes), with the next
+//      bit's statep read issued before the current write and forwarded when
+//      both hit the same state;
+//   4. the final state of every touched slot back to HBM.
+__device__ __forceinline__ void update_unit(const WaveMem &M, const LDS DevTables *T, GLB uint8_t *slot, uint32_t tb,
+                            uint32_t Umax, uint32_t nvalid, uint32_t lane, PhaseClock &clk) {
   const uint32_t TS = table_size(Umax);
+  const uint32_t nbytes = nvalid >> 3;
+  // y of every unit bit as a uniform bit mask
+  uint64_t ym[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t r = lane + 64u * j;
+    const bool y = (r < nvalid) && (M.rprm[r < Umax ? r : 0] & 1u);
+    ym[j] = __ballot(y);
+  }
+  // 1. mixer deltas
   for (uint32_t r = lane; r < nvalid; r += 64) {
     const uint32_t prm = M.rprm[r];
     const int32_t err = ((int32_t)((prm & 1u) << 12) - (int32_t)(prm >> 1)) * 7;
     const uint32_t cxt = M.rcxt[r];
+    int32_t d[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const uint32_t g = M.rg[k * Umax + r];
       const int32_t smp = T->stretch[M.stp[k * 256 + g] >> 20];
-      const int32_t d = (int32_t)(((int64_t)err * smp) >> 16);
-      atomicAdd(&M.w[cxt * 8 + k], d);
+      d[k] = (int32_t)(((int64_t)err * smp) >> 16);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) atomicAdd((int32_t *)&M.w[cxt * 8 + k], d[k]);
+  }
+  // bytes with an in-byte slot collision in some order >= 1
+  uint32_t cmask = 0;
+  for (uint32_t base = 0; base < 7 * nbytes; base += 64) {
+    const uint32_t it = base + lane;
+    bool coll = false;
+    if (it < 7 * nbytes) {
+      const uint32_t b = it / 7, k = 1 + it % 7;
+      const u32x4 v = *(const LDS u32x4 *)(M.ridx + k * Umax + 8 * b);
+      const uint32_t e[8] = {v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16,
+                             v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16};
+#pragma unroll
+      for (int x = 0; x < 8; ++x)
+#pragma unroll
+        for (int z = x + 1; z < 8; ++z) coll |= (e[x] == e[z]);
+    }
+    uint64_t bal = __ballot(coll);
+    while (bal) {
+      const uint32_t bit = (uint32_t)__builtin_ctzll(bal);
+      bal &= bal - 1;
+      cmask |= 1u << ((base + bit) / 7);
     }
   }
   wave_sync();
-  if (lane < 8) {
-    const uint32_t k = lane;
-    uint32_t *sp = M.stp + k * 256;
-    const uint16_t *ri = M.ridx + k * Umax;
-    const uint8_t *rgk = M.rg + k * Umax;
-    uint16_t *tv = M.tval + (k == 0 ? 0 : (k - 1) * TS);
-    for (uint32_t r = 0; r < nvalid; ++r) {
-      const uint32_t y = M.rprm[r] & 1u;
-      const uint32_t i = ri[r];
-      uint32_t s;
-      if (k == 0) {
-        s = M.st0[i];
-      } else {
-        const uint32_t v = tv[i];
-        s = (v > 255u) ? (uint32_t)rgk[r] : v;
+  clk.mark(5);
+  // 2. state resolution, one byte of the unit per step
+  {
+    const uint32_t k = lane >> 3, j = lane & 7u;
+    LDS uint16_t *tv = M.tval + (k ? (k - 1) * TS : 0);
+#pragma unroll
+    for (int seg = 0; seg < 4; ++seg) {
+      const uint64_t yseg = ym[seg];
+      const uint32_t bend = min(nbytes, 8u * seg + 8u);
+      for (uint32_t b = 8u * seg; b < bend; ++b) {
+        const uint32_t r = 8 * b + j;
+        const uint32_t y = (uint32_t)(yseg >> (r & 63)) & 1u;
+        const uint32_t i = M.ridx[k * Umax + r];
+        const uint32_t g = M.rg[k * Umax + r];
+        const uint32_t reps = ((cmask >> b) & 1u) ? 8u : 1u;
+        for (uint32_t q = 0; q < reps; ++q) {
+          if (reps == 1u || q == j) {
+            uint32_t st;
+            if (k == 0) {
+              st = M.st0[i];
+            } else {
+              const uint32_t v = tv[i];
+              st = v > 255u ? g : v;
+            }
+            const uint32_t ns = T->nex[2 * st + y];
+            if (k == 0)
+              M.st0[i] = (uint8_t)ns;
+            else
+              tv[i] = (uint16_t)ns;
+            M.rg[k * Umax + r] = (uint8_t)st;
+          }
+          wave_sync();
+        }
       }
-      const uint32_t p = sp[s];
-      const uint32_t n = p & 1023u;
-      const int32_t e = (int32_t)((y << 22) - (p >> 10));
-      const uint32_t adj = ((uint32_t)(e >> 3) * (uint32_t)T->dt[n]) & 0xfffffc00u;
-      sp[s] = p + adj + (n < 1023u ? 1u : 0u);
-      const uint32_t ns = T->nex[2 * s + y];
-      if (k == 0)
-        M.st0[i] = (uint8_t)ns;
-      else
-        tv[i] = (uint16_t)ns;
+    }
+  }
+  clk.mark(6);
+  // 3. statep recurrences
+  if (lane < 8) {
+    LDS uint32_t *sp = M.stp + lane * 256;
+    const LDS uint8_t *S = M.rg + lane * Umax;
+    uint32_t s = S[0];
+    uint32_t p = sp[s];
+#pragma unroll
+    for (int seg = 0; seg < 4; ++seg) {
+      const uint64_t yseg = ym[seg];
+      const uint32_t rend = min(nvalid, 64u * seg + 64u);
+      for (uint32_t r = 64u * seg; r < rend; ++r) {
+        const uint32_t sn = (r + 1 < nvalid) ? S[r + 1] : s;
+        const uint32_t pn_raw = sp[sn];
+        const uint32_t y = (uint32_t)(yseg >> (r & 63)) & 1u;
+        const uint32_t n = p & 1023u;
+        const int32_t e = (int32_t)((y << 22) - (p >> 10));
+        const uint32_t adj = ((uint32_t)(e >> 3) * adapt_rate(n)) & 0xfffffc00u;
+        const uint32_t pn = p + adj + (n < 1023u ? 1u : 0u);
+        sp[s] = pn;
+        p = (sn == s) ? pn : pn_raw;
+        s = sn;
+      }
     }
   }
   wave_sync();
-  for (uint32_t e = lane; e < 7 * TS; e += 64) {
-    const uint32_t key = M.tkey[e];
-    if (key != kEmpty) {
-      const uint32_t kk = e / TS;
-      slot[((uint64_t)kk << tb) + key] = (uint8_t)M.tval[e];
-      M.tkey[e] = kEmpty;
-      M.tval[e] = kNoState;
+  clk.mark(7);
+  // 4. write back the touched slots (every occurrence writes the same final
+  // state), then clear the tables
+#pragma unroll 1
+  for (uint32_t k = 1; k < 8; ++k) {
+    GLB uint8_t *tab = slot + ((uint64_t)(k - 1) << tb);
+    for (uint32_t r = lane; r < nvalid; r += 64) {
+      const uint32_t i = M.ridx[k * Umax + r];
+      tab[M.tkey[(k - 1) * TS + i]] = (uint8_t)M.tval[(k - 1) * TS + i];
     }
   }
-  global_sync();
   wave_sync();
+#pragma unroll 1
+  for (uint32_t k = 1; k < 8; ++k)
+    for (uint32_t r = lane; r < nvalid; r += 64) {
+      const uint32_t i = M.ridx[k * Umax + r];
+      M.tkey[(k - 1) * TS + i] = kEmpty;
+      M.tval[(k - 1) * TS + i] = kNoState;
+    }
+  vm_drain();
+  wave_sync();
+  clk.mark(8);
 }
 
 // ---------------------------------------------------------------------------
 // Encoder
 // ---------------------------------------------------------------------------
 
-__device__ void encode_task(const CodecArgs &a, const Task &tk, uint32_t t, const WaveMem &M,
-                            const DevTables *T, uint8_t *slot, uint32_t Umax, uint32_t lane) {
+__device__ __forceinline__ void encode_task(const CodecArgs &a, const Task &tk, uint32_t t, const WaveMem &M,
+                            const LDS DevTables *T, GLB uint8_t *slot, uint32_t Umax, uint32_t lane) {
   const uint32_t U = tk.U, tb = tk.tb, mask = (1u << tb) - 1u;
   const uint32_t TS = table_size(Umax);
   const uint64_t L = tk.L;
-  const uint8_t *src = (const uint8_t *)(uintptr_t)tk.src;
-  uint8_t *dst = (uint8_t *)(uintptr_t)tk.dst;
+  const GLB uint8_t *src = (const GLB uint8_t *)(uintptr_t)tk.src;
+  GLB uint8_t *dst = (GLB uint8_t *)(uintptr_t)tk.dst;
   const uint64_t cap = tk.aux;
+  PhaseClock clk(a.prof != nullptr);
   init_model(a, tk, M, slot, Umax, lane);
+  clk.mark(0);
   const uint64_t vbits = 8 * L;
   const uint64_t coded_end = (tk.flags & kTaskTailPad) ? 8ull * tk.S : vbits;
   uint32_t x1 = 0, x2 = 0xffffffffu, acc = 0;
@@ -269,67 +440,95 @@ __device__ void encode_task(const CodecArgs &a, const Task &tk, uint32_t t, cons
   for (uint64_t base = 0; base < coded_end; base += U) {
     const uint32_t nvalid = base < vbits ? (uint32_t)min((uint64_t)U, vbits - base) : 0u;
     if (nvalid) {
-      // Stage the 8 history bytes and the unit's bytes (zero outside the chunk).
+      // 8 history bytes and the unit's bytes (zero outside the chunk)
       const int64_t b0 = (int64_t)(base >> 3) - 8;
       for (uint32_t i = lane; i < 8 + U / 8; i += 64) {
         const int64_t p = b0 + i;
         M.ub[i] = (p >= 0 && p < (int64_t)L) ? src[p] : (uint8_t)0;
       }
       wave_sync();
-      for (uint32_t r = lane; r < nvalid; r += 64) {
-        const uint32_t bi = r >> 3, j = r & 7u;
-        const uint32_t cur = M.ub[8 + bi];
-        const uint32_t h0 = (1u << j) | (j ? (cur >> (8 - j)) : 0u);
-        uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
-        uint32_t sl[8];
+      clk.mark(1);
+      // predict: lane handles unit bits r = lane + 64j; gathers of all its
+      // bits are in flight together
+      uint32_t sl[4][8], g[4][8];
 #pragma unroll
-        for (int k = 1; k < 8; ++k) {
-          H = (H ^ (uint32_t)M.ub[8 + bi - k]) * kFnvPrime;
-          sl[k] = H & mask;
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t r = lane + 64u * j;
+        if (r < nvalid) {
+          const uint32_t bi = r >> 3, jb = r & 7u;
+          const uint32_t cur = M.ub[8 + bi];
+          const uint32_t h0 = (1u << jb) | (jb ? (cur >> (8 - jb)) : 0u);
+          uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
+          sl[j][0] = h0;
+#pragma unroll
+          for (int k = 1; k < 8; ++k) {
+            H = (H ^ (uint32_t)M.ub[8 + bi - k]) * kFnvPrime;
+            sl[j][k] = H & mask;
+          }
+          g[j][0] = M.st0[h0];
+#pragma unroll
+          for (int k = 1; k < 8; ++k) g[j][k] = slot[((uint64_t)(k - 1) << tb) + sl[j][k]];
         }
-        uint32_t g[8];
-        g[0] = M.st0[h0];
+      }
+      if (clk.on) __builtin_amdgcn_s_waitcnt(0x0f70);  // attribute the gather wait to phase 2
+      clk.mark(2);
 #pragma unroll
-        for (int k = 1; k < 8; ++k) g[k] = slot[((uint64_t)(k - 1) << tb) + sl[k]];
-        M.ridx[r] = (uint16_t)h0;
-        M.rg[r] = (uint8_t)g[0];
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t r = lane + 64u * j;
+        if (r < nvalid) {
+          uint32_t idx[8];
+          insert_slots(M.tkey, TS, sl[j], idx);
+          M.ridx[r] = (uint16_t)sl[j][0];
+          M.rg[r] = (uint8_t)g[j][0];
 #pragma unroll
-        for (int k = 1; k < 8; ++k) {
-          M.ridx[k * Umax + r] = (uint16_t)table_insert(M.tkey + (k - 1) * TS, TS, sl[k]);
-          M.rg[k * Umax + r] = (uint8_t)g[k];
+          for (int k = 1; k < 8; ++k) {
+            M.ridx[k * Umax + r] = (uint16_t)idx[k];
+            M.rg[k * Umax + r] = (uint8_t)g[j][k];
+          }
+          int32_t smp[8];
+          uint32_t active = 0;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            smp[k] = T->stretch[M.stp[k * 256 + g[j][k]] >> 20];
+            active += (smp[k] != 1);
+          }
+          const uint32_t bi = r >> 3, jb = r & 7u;
+          const uint32_t cxt = active + (uint32_t)(M.ub[8 + bi - 1] >> 5) * 10u;
+          const uint32_t pr = mix_predict(M.w + cxt * 8, smp, T->squash);
+          const uint32_t y = ((uint32_t)M.ub[8 + bi] >> (7 - jb)) & 1u;
+          M.rprm[r] = (uint16_t)((pr << 1) | y);
+          M.rcxt[r] = (uint8_t)cxt;
         }
-        int32_t smp[8];
-        uint32_t active = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          smp[k] = T->stretch[M.stp[k * 256 + g[k]] >> 20];
-          active += (smp[k] != 1);
-        }
-        const uint32_t cxt = active + (uint32_t)(M.ub[8 + bi - 1] >> 5) * 10u;
-        const uint32_t pr = mix_predict(M.w + cxt * 8, smp, T->squash);
-        const uint32_t y = (cur >> (7 - j)) & 1u;
-        M.rprm[r] = (uint16_t)((pr << 1) | y);
-        M.rcxt[r] = (uint8_t)cxt;
       }
       wave_sync();
+      clk.mark(3);
     }
-    if (lane == 0) {
-      for (uint32_t r = 0; r < U; ++r) {
-        if (base + r >= coded_end) break;
-        const uint32_t prm = M.rprm[r];
+    // Arithmetic coder: a uniform (scalar) loop; (pr, y) of unit bit r is
+    // read from lane r & 63 of the register that holds bits 64j..64j+63.
+    uint32_t pv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) pv[j] = (lane + 64u * j < Umax) ? M.rprm[lane + 64u * j] : 0u;
+    const uint32_t nc = (uint32_t)min((uint64_t)U, coded_end - base);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t lo = 64u * j;
+      const uint32_t hi = min(nc, lo + 64u);
+      for (uint32_t r = lo; r < hi; ++r) {
+        const uint32_t prm = __builtin_amdgcn_readlane(pv[j], r - lo);
         const uint32_t xmid = coder_mid(x1, x2, prm >> 1);
         if (prm & 1u)
           x2 = xmid;
         else
           x1 = xmid + 1u;
         while (((x1 ^ x2) & 0xff000000u) == 0) {
-          acc |= (x2 >> 24) << (8 * (opos & 3));
+          acc |= (x2 >> 24) << (8 * (uint32_t)(opos & 3));
           ++opos;
           if ((opos & 3) == 0) {
-            if (opos <= cap)
-              *(uint32_t *)(dst + opos - 4) = acc;
-            else
+            if (opos <= cap) {
+              if (lane == 0) *(GLB uint32_t *)(dst + opos - 4) = acc;
+            } else {
               ovf = true;
+            }
             acc = 0;
           }
           x1 <<= 8;
@@ -337,19 +536,20 @@ __device__ void encode_task(const CodecArgs &a, const Task &tk, uint32_t t, cons
         }
       }
     }
-    if (nvalid) update_unit(M, T, slot, tb, Umax, nvalid, lane);
+    clk.mark(4);
+    if (nvalid) update_unit(M, T, slot, tb, Umax, nvalid, lane, clk);
     wave_sync();
   }
-  if (lane == 0) {
-    acc |= (x1 >> 24) << (8 * (opos & 3));
-    ++opos;
-    const uint64_t wstart = (opos - 1) & ~3ull;
-    if (wstart + 4 <= cap)
-      *(uint32_t *)(dst + wstart) = acc;
-    else
-      ovf = true;
-    a.result[t] = opos | (ovf ? kResultOverflow : 0ull);
+  acc |= (x1 >> 24) << (8 * (uint32_t)(opos & 3));
+  ++opos;
+  const uint64_t wstart = (opos - 1) & ~3ull;
+  if (wstart + 4 <= cap) {
+    if (lane == 0) *(GLB uint32_t *)(dst + wstart) = acc;
+  } else {
+    ovf = true;
   }
+  if (lane == 0) a.result[t] = opos | (ovf ? kResultOverflow : 0ull);
+  clk.flush(a.prof, lane);
   wave_sync();
 }
 
@@ -357,39 +557,60 @@ __device__ void encode_task(const CodecArgs &a, const Task &tk, uint32_t t, cons
 // Decoder
 // ---------------------------------------------------------------------------
 
-__device__ void decode_task(const CodecArgs &a, const Task &tk, uint32_t t, const WaveMem &M,
-                            const DevTables *T, uint8_t *slot, uint32_t Umax, uint32_t lane) {
+// Window of the payload in registers: lane l holds bytes 4l..4l+3 after
+// winbase (0xff past the payload, as the reference decoder reads).
+__device__ __forceinline__ uint32_t load_window(const GLB uint8_t *pay, uint64_t plen, uint64_t wb,
+                                                uint32_t lane) {
+  uint32_t w = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint64_t at = wb + 4ull * lane + i;
+    w |= (uint32_t)(at < plen ? pay[at] : (uint8_t)255) << (8 * i);
+  }
+  return w;
+}
+
+__device__ __forceinline__ uint32_t window_byte(uint32_t win, uint64_t off) {
+  const uint32_t w = __builtin_amdgcn_readlane(win, (uint32_t)(off >> 2));
+  return (w >> (8 * (uint32_t)(off & 3))) & 255u;
+}
+
+__device__ __forceinline__ uint32_t lane_value(const uint32_t v[4], uint32_t q) {
+  if (q < 64) return __builtin_amdgcn_readlane(v[0], q);
+  if (q < 128) return __builtin_amdgcn_readlane(v[1], q - 64);
+  if (q < 192) return __builtin_amdgcn_readlane(v[2], q - 128);
+  return __builtin_amdgcn_readlane(v[3], q - 192);
+}
+
+__device__ __forceinline__ void decode_task(const CodecArgs &a, const Task &tk, uint32_t t, const WaveMem &M,
+                            const LDS DevTables *T, GLB uint8_t *slot, uint32_t Umax, uint32_t lane) {
   const uint32_t U = tk.U, tb = tk.tb, mask = (1u << tb) - 1u;
   const uint32_t TS = table_size(Umax);
   const uint32_t D = a.spec_bits, P = (1u << D) - 1u;
   const uint64_t L = tk.L;
-  const uint8_t *pay = (const uint8_t *)(uintptr_t)tk.src;
+  const GLB uint8_t *pay = (const GLB uint8_t *)(uintptr_t)tk.src;
   const uint64_t plen = tk.aux;
-  uint8_t *dst = (uint8_t *)(uintptr_t)tk.dst;
+  GLB uint8_t *dst = (GLB uint8_t *)(uintptr_t)tk.dst;
+  PhaseClock clk(a.prof != nullptr);
   init_model(a, tk, M, slot, Umax, lane);
+  clk.mark(0);
   uint64_t winbase = 0;
-  for (uint32_t i = lane; i < kWindow; i += 64) M.win[i] = (i < plen) ? pay[i] : (uint8_t)255;
-  wave_sync();
-  uint32_t x1 = 0, x2 = 0xffffffffu, x = 0;
+  uint32_t win = load_window(pay, plen, 0, lane);
+  uint32_t x = __builtin_bswap32(__builtin_amdgcn_readlane(win, 0));
+  uint32_t x1 = 0, x2 = 0xffffffffu;
   uint64_t ip = 4;
-  if (lane == 0)
-    x = ((uint32_t)M.win[0] << 24) | ((uint32_t)M.win[1] << 16) | ((uint32_t)M.win[2] << 8) |
-        (uint32_t)M.win[3];
   const uint64_t vbits = 8 * L;
   for (uint64_t base = 0; base < vbits; base += U) {
     const uint32_t nbits = (uint32_t)min((uint64_t)U, vbits - base);
     for (uint32_t rb = 0; rb < nbits; rb += D) {
-      // Keep the coder's next bytes in the LDS window (<= 4 bytes per bit).
-      const uint64_t ipu = __shfl(ip, 0);
-      if (ipu + 4 * D + 8 > winbase + kWindow) {
-        winbase = ipu;
-        for (uint32_t i = lane; i < kWindow; i += 64)
-          M.win[i] = (winbase + i < plen) ? pay[winbase + i] : (uint8_t)255;
-        wave_sync();
+      // the coder consumes at most 4 bytes per bit
+      if (ip + 4 * D + 8 > winbase + 256) {
+        winbase = ip & ~3ull;
+        win = load_window(pay, plen, winbase, lane);
       }
       const uint32_t j0 = rb & 7u, bi = rb >> 3;
       const uint32_t known = j0 ? ((uint32_t)M.ub[8 + bi] >> (8 - j0)) : 0u;
-      // Speculative predictions: path q fixes the next plen(q) bits to pbits(q).
+      // speculative predictions: path q fixes the next plen(q) bits to pbits(q)
       for (uint32_t it = lane; it < 8 * P; it += 64) {
         const uint32_t q = it >> 3, k = it & 7u;
         const uint32_t plen_q = 31u - __clz(q + 1);
@@ -411,58 +632,70 @@ __device__ void decode_task(const CodecArgs &a, const Task &tk, uint32_t t, cons
         M.psmp[q * 8 + k] = T->stretch[M.stp[k * 256 + g] >> 20];
       }
       wave_sync();
-      for (uint32_t q = lane; q < P; q += 64) {
-        const uint32_t plen_q = 31u - __clz(q + 1);
-        if (rb + plen_q >= nbits) continue;
-        int32_t smp[8];
-        uint32_t active = 0;
+      clk.mark(9);
+      // mix every path (lane q + 64m holds path q + 64m)
+      uint32_t prq[4] = {0, 0, 0, 0}, cxq[4] = {0, 0, 0, 0};
+      const uint32_t c1 = M.ub[8 + bi - 1] >> 5;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          smp[k] = M.psmp[q * 8 + k];
-          active += (smp[k] != 1);
-        }
-        const uint32_t cxt = active + (uint32_t)(M.ub[8 + bi - 1] >> 5) * 10u;
-        M.ppr[q] = (uint16_t)mix_predict(M.w + cxt * 8, smp, T->squash);
-        M.pcxt[q] = (uint8_t)cxt;
-      }
-      wave_sync();
-      const uint32_t nd = min(D, nbits - rb);
-      if (lane == 0) {
-        uint32_t pre = 0;
-        for (uint32_t i = 0; i < nd; ++i) {
-          const uint32_t q = (1u << i) - 1u + pre;
-          const uint32_t pr = M.ppr[q];
-          const uint32_t xmid = coder_mid(x1, x2, pr);
-          const uint32_t y = x <= xmid;
-          if (y)
-            x2 = xmid;
-          else
-            x1 = xmid + 1u;
-          while (((x1 ^ x2) & 0xff000000u) == 0) {
-            x1 <<= 8;
-            x2 = (x2 << 8) | 255u;
-            x = (x << 8) | (uint32_t)M.win[ip - winbase];
-            ++ip;
+      for (int m = 0; m < 4; ++m) {
+        const uint32_t q = lane + 64u * m;
+        if (q < P) {
+          const uint32_t plen_q = 31u - __clz(q + 1);
+          if (rb + plen_q < nbits) {
+            int32_t smp[8];
+            uint32_t active = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              smp[k] = M.psmp[q * 8 + k];
+              active += (smp[k] != 1);
+            }
+            const uint32_t cxt = active + c1 * 10u;
+            prq[m] = mix_predict(M.w + cxt * 8, smp, T->squash);
+            cxq[m] = cxt;
           }
-          pre = (pre << 1) | y;
-          M.chosen[i] = (uint8_t)q;
-          M.rprm[rb + i] = (uint16_t)((pr << 1) | y);
-          M.rcxt[rb + i] = M.pcxt[q];
         }
-        M.ub[8 + bi] = (uint8_t)(M.ub[8 + bi] | (pre << (8 - j0 - nd)));
       }
-      wave_sync();
+      // decode up to D bits (uniform scalar loop)
+      const uint32_t nd = min(D, nbits - rb);
+      uint32_t pre = 0;
+      uint64_t chosen = 0;
+      for (uint32_t i = 0; i < nd; ++i) {
+        const uint32_t q = (1u << i) - 1u + pre;
+        const uint32_t pr = lane_value(prq, q);
+        const uint32_t cx = lane_value(cxq, q);
+        const uint32_t xmid = coder_mid(x1, x2, pr);
+        const uint32_t y = x <= xmid;
+        if (y)
+          x2 = xmid;
+        else
+          x1 = xmid + 1u;
+        while (((x1 ^ x2) & 0xff000000u) == 0) {
+          x1 <<= 8;
+          x2 = (x2 << 8) | 255u;
+          x = (x << 8) | window_byte(win, ip - winbase);
+          ++ip;
+        }
+        pre = (pre << 1) | y;
+        chosen |= (uint64_t)q << (8 * i);
+        if (lane == 0) {
+          M.rprm[rb + i] = (uint16_t)((pr << 1) | y);
+          M.rcxt[rb + i] = (uint8_t)cx;
+        }
+      }
+      if (lane == 0) M.ub[8 + bi] = (uint8_t)(M.ub[8 + bi] | (pre << (8 - j0 - nd)));
+      clk.mark(10);
+      // record the taken paths' slots and unit-start states
       if (lane < 8 * nd) {
         const uint32_t i = lane >> 3, k = lane & 7u, r = rb + i;
-        const uint32_t q = M.chosen[i];
+        const uint32_t q = (uint32_t)(chosen >> (8 * i)) & 255u;
         M.rg[k * Umax + r] = M.pg[q * 8 + k];
         const uint32_t sl = M.pslot[q * 8 + k];
-        M.ridx[k * Umax + r] =
-            (uint16_t)(k == 0 ? sl : table_insert(M.tkey + (k - 1) * TS, TS, sl));
+        M.ridx[k * Umax + r] = (uint16_t)(k == 0 ? sl : insert_slot(M.tkey + (k - 1) * TS, TS, sl));
       }
       wave_sync();
+      clk.mark(11);
     }
-    update_unit(M, T, slot, tb, Umax, nbits, lane);
+    update_unit(M, T, slot, tb, Umax, nbits, lane, clk);
     const uint32_t nb = nbits >> 3;
     for (uint32_t i = lane; i < nb; i += 64) dst[(base >> 3) + i] = M.ub[8 + i];
     uint8_t hist = 0;
@@ -473,6 +706,7 @@ __device__ void decode_task(const CodecArgs &a, const Task &tk, uint32_t t, cons
     wave_sync();
   }
   if (lane == 0) a.result[t] = ip;
+  clk.flush(a.prof, lane);
   wave_sync();
 }
 
@@ -480,10 +714,10 @@ __device__ void decode_task(const CodecArgs &a, const Task &tk, uint32_t t, cons
 // Kernels
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void stage_tables(const DevTables *g, DevTables *s) {
+__device__ __forceinline__ void stage_tables(const DevTables *g, LDS DevTables *s) {
   const uint32_t n = sizeof(DevTables) / 16;
-  const uint4 *src = (const uint4 *)g;
-  uint4 *dst = (uint4 *)s;
+  const u32x4 *src = (const u32x4 *)g;
+  LDS u32x4 *dst = (LDS u32x4 *)s;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
   __syncthreads();
 }
@@ -491,20 +725,19 @@ __device__ __forceinline__ void stage_tables(const DevTables *g, DevTables *s) {
 template <bool kDecode>
 __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a, uint32_t Umax) {
   extern __shared__ __align__(16) uint8_t lds[];
-  DevTables *T = (DevTables *)lds;
+  LDS DevTables *T = (LDS DevTables *)lds;
   stage_tables(a.tabs, T);
   const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t P = kDecode ? (1u << a.spec_bits) - 1u : 0u;
   const uint32_t wbytes = wave_lds_bytes(Umax, P, kDecode);
-  uint8_t *wbase = lds + align16(sizeof(DevTables)) + wid * wbytes;
-  WaveMem M;
-  wave_lds_bytes(Umax, P, kDecode, &M, wbase);
+  LDS uint8_t *wbase = (LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * wbytes;
+  const WaveMem M = carve(wbase, Umax, P, kDecode);
   const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wid;
-  uint8_t *slot = a.pool + gw * a.slot_bytes;
+  GLB uint8_t *slot = (GLB uint8_t *)a.pool + gw * a.slot_bytes;
   for (;;) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(a.counter, 1u);
-    t = __shfl(t, 0);
+    t = __builtin_amdgcn_readfirstlane(t);
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
     if (kDecode)

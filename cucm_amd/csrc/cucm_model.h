@@ -101,6 +101,9 @@ struct DevInitModel {
 // bit 63 set if the staging region overflowed.
 constexpr uint64_t kResultOverflow = 1ull << 63;
 
+// Phases of the per-phase cycle counters (cucm_ctx_phase_cycles).
+constexpr int kPhases = 12;
+
 struct CodecArgs {
   const Task *tasks;
   uint32_t ntasks;
@@ -109,6 +112,7 @@ struct CodecArgs {
   uint8_t *pool;           // model slots, one per resident wave
   uint64_t slot_bytes;     // bytes per model slot (7 << max tb)
   uint64_t *result;        // per task
+  uint64_t *prof;          // optional per-phase cycle totals (kPhases), else null
   const DevTables *tabs;
   const DevInitModel *init;
 };

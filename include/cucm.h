@@ -149,6 +149,18 @@ typedef struct cucm_stats {
 } cucm_stats;
 int cucm_ctx_last_stats(cucm_ctx *ctx, cucm_stats *st);
 
+/* Per-phase shader-clock totals, summed over all waves, of the last codec
+ * launch on ctx (phase profiling must be enabled first; it adds a clock read
+ * per phase).  The counterpart of the reference's host phase timers
+ * g_time_mem/predict/mix/encode (cucm_compress_v2.cu:983-988).  Phases: 0 model
+ * init, 1 unit staging, 2 hash + gather, 3 slot tables + mix, 4 arithmetic
+ * coder, 5 mixer deltas + collision scan, 6 state resolution, 7 statep
+ * recurrences, 8 state write-back, 9 decoder path predictions, 10 decoder mix +
+ * decode, 11 decoder records. */
+#define CUCM_PHASES 12
+int cucm_ctx_set_phase_profile(cucm_ctx *ctx, int enable);
+int cucm_ctx_phase_cycles(cucm_ctx *ctx, uint64_t cycles[CUCM_PHASES]);
+
 /* Upper bound on concurrently resident worker waves (0 = automatic). */
 int cucm_ctx_set_max_waves(cucm_ctx *ctx, uint32_t waves);
 
