@@ -70,12 +70,13 @@ struct WaveMem {
   LDS uint8_t *ub;      // [8 + Umax/8] 8 history bytes + the unit's bytes
   LDS uint8_t *win;     // [kWindow] decoder payload window
   LDS uint8_t *chosen;  // [8] decoder: path taken by each bit of a round
+  LDS uint32_t *dummy;  // [64] scratch target of predicated-off LDS operations
 };
 
 // Byte offsets of the per-wave LDS arrays (16-byte aligned each).
 struct WaveLayout {
   uint32_t stp, w, tkey, pslot, tval, ridx, rprm, psmp, ppr, st0, rg, rcxt, pg, pcxt, ub, win,
-      chosen, total;
+      chosen, dummy, total;
 };
 
 __host__ __device__ __forceinline__ WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec) {
@@ -103,6 +104,7 @@ __host__ __device__ __forceinline__ WaveLayout wave_layout(uint32_t Umax, uint32
   CUCM_TAKE(ub, 8 + Umax / 8 + 8)
   CUCM_TAKE(win, dec ? kWindow : 0u)
   CUCM_TAKE(chosen, dec ? 8u : 0u)
+  CUCM_TAKE(dummy, 64 * 4)
 #undef CUCM_TAKE
   l.total = off;
   return l;
@@ -132,6 +134,7 @@ __device__ __forceinline__ WaveMem carve(LDS uint8_t *base, uint32_t Umax, uint3
   m.ub = base + l.ub;
   m.win = base + l.win;
   m.chosen = base + l.chosen;
+  m.dummy = (LDS uint32_t *)(base + l.dummy);
   return m;
 }
 
@@ -190,6 +193,19 @@ __device__ __forceinline__ uint32_t mix_predict(const LDS int32_t *wrow, const i
   return pr;
 }
 
+// Mixer::p with the weight row already in registers.
+__device__ __forceinline__ uint32_t mix_weights(const int32_t w[8], const int32_t smp[8],
+                                                const LDS uint16_t *squash) {
+  int64_t dot = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dot += (int64_t)w[k] * (int64_t)smp[k];
+  dot >>= 16;
+  const int idx = dot < -2047 ? -2047 : (dot > 2048 ? 2048 : (int)dot);
+  uint32_t pr = squash[idx + 2047];
+  if (pr < 2048) ++pr;
+  return pr;
+}
+
 // Carry-less coder interval split (cucm_compress_v2.cu:855-857).
 __device__ __forceinline__ uint32_t coder_mid(uint32_t x1, uint32_t x2, uint32_t pr) {
   const uint32_t rng = x2 - x1;
@@ -207,31 +223,93 @@ __device__ __forceinline__ uint32_t adapt_rate(uint32_t n) {
   return q;
 }
 
-// Inserts the 7 slots of orders 1..7 into the unit's distinct-slot tables;
-// the first probe of every order is issued before any result is examined.
-__device__ __forceinline__ void insert_slots(LDS uint32_t *tkey, uint32_t TS, const uint32_t sl[8],
-                                             uint32_t idx[8]) {
-  uint32_t old[8];
+// Distinct-slot tables (one open-addressing table of TS keys per order
+// 1..7): places every pending (j, k) key and leaves its bucket in idx.
+// Returning LDS atomics are slow on this hardware, so insertion runs in
+// rounds of plain reads and writes: read the bucket; my key -> done; empty
+// -> write my key and read it back (the last writer wins, losers probe on);
+// another key -> probe the next bucket.  Lanes holding the same key see the
+// same values in every round, so they move together and a key is placed in
+// exactly one bucket.  Each round fills every contested empty bucket, and a
+// table has at least twice as many buckets as keys, so the loop ends.
+template <int NJ>
+__device__ __forceinline__ void place_slots(LDS uint32_t *tkey, uint32_t TS,
+                                            const uint32_t sl[NJ][8], uint32_t idx[NJ][8],
+                                            uint32_t pend) {
+  while (__ballot(pend != 0)) {
+    uint32_t v[NJ][8];
 #pragma unroll
-  for (int k = 1; k < 8; ++k) {
-    idx[k] = sl[k] & (TS - 1);
-    old[k] = atomicCAS((uint32_t *)&tkey[(k - 1) * TS + idx[k]], kEmpty, sl[k]);
-  }
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
-  for (int k = 1; k < 8; ++k) {
-    while (old[k] != kEmpty && old[k] != sl[k]) {
-      idx[k] = (idx[k] + 1) & (TS - 1);
-      old[k] = atomicCAS((uint32_t *)&tkey[(k - 1) * TS + idx[k]], kEmpty, sl[k]);
-    }
+      for (int k = 1; k < 8; ++k)
+        if (pend & (1u << (j * 8 + k))) v[j][k] = tkey[(k - 1) * TS + idx[j][k]];
+    uint32_t cand = 0;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        const uint32_t bit = 1u << (j * 8 + k);
+        if (pend & bit) {
+          if (v[j][k] == sl[j][k]) {
+            pend &= ~bit;
+          } else if (v[j][k] == kEmpty) {
+            tkey[(k - 1) * TS + idx[j][k]] = sl[j][k];
+            cand |= bit;
+          } else {
+            idx[j][k] = (idx[j][k] + 1) & (TS - 1);
+          }
+        }
+      }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int k = 1; k < 8; ++k)
+        if (cand & (1u << (j * 8 + k))) v[j][k] = tkey[(k - 1) * TS + idx[j][k]];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        const uint32_t bit = 1u << (j * 8 + k);
+        if (cand & bit) {
+          if (v[j][k] == sl[j][k])
+            pend &= ~bit;
+          else
+            idx[j][k] = (idx[j][k] + 1) & (TS - 1);
+        }
+      }
+    wave_sync();
   }
 }
 
-__device__ __forceinline__ uint32_t insert_slot(LDS uint32_t *tk, uint32_t TS, uint32_t slot) {
-  uint32_t h = slot & (TS - 1);
-  uint32_t old = atomicCAS((uint32_t *)&tk[h], kEmpty, slot);
-  while (old != kEmpty && old != slot) {
-    h = (h + 1) & (TS - 1);
-    old = atomicCAS((uint32_t *)&tk[h], kEmpty, slot);
+// Single-key form of place_slots for lanes holding one key of order k (k >= 1);
+// lanes with active == false take part in the rounds without a key.
+__device__ __forceinline__ uint32_t place_slot(LDS uint32_t *tk, uint32_t TS, uint32_t key,
+                                               bool active) {
+  uint32_t h = key & (TS - 1);
+  bool pend = active;
+  while (__ballot(pend)) {
+    uint32_t v = 0;
+    if (pend) v = tk[h];
+    bool cand = false;
+    if (pend) {
+      if (v == key) {
+        pend = false;
+      } else if (v == kEmpty) {
+        tk[h] = key;
+        cand = true;
+      } else {
+        h = (h + 1) & (TS - 1);
+      }
+    }
+    wave_sync();
+    if (cand) {
+      if (tk[h] == key)
+        pend = false;
+      else
+        h = (h + 1) & (TS - 1);
+    }
+    wave_sync();
   }
   return h;
 }
@@ -281,14 +359,15 @@ __device__ __forceinline__ void init_model(const CodecArgs &a, const Task &tk, c
 //      bit's statep read issued before the current write and forwarded when
 //      both hit the same state;
 //   4. the final state of every touched slot back to HBM.
+template <int NJ>
 __device__ __forceinline__ void update_unit(const WaveMem &M, const LDS DevTables *T, GLB uint8_t *slot, uint32_t tb,
                             uint32_t Umax, uint32_t nvalid, uint32_t lane, PhaseClock &clk) {
   const uint32_t TS = table_size(Umax);
   const uint32_t nbytes = nvalid >> 3;
   // y of every unit bit as a uniform bit mask
-  uint64_t ym[4];
+  uint64_t ym[NJ];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < NJ; ++j) {
     const uint32_t r = lane + 64u * j;
     const bool y = (r < nvalid) && (M.rprm[r < Umax ? r : 0] & 1u);
     ym[j] = __ballot(y);
@@ -337,7 +416,7 @@ __device__ __forceinline__ void update_unit(const WaveMem &M, const LDS DevTable
     const uint32_t k = lane >> 3, j = lane & 7u;
     LDS uint16_t *tv = M.tval + (k ? (k - 1) * TS : 0);
 #pragma unroll
-    for (int seg = 0; seg < 4; ++seg) {
+    for (int seg = 0; seg < NJ; ++seg) {
       const uint64_t yseg = ym[seg];
       const uint32_t bend = min(nbytes, 8u * seg + 8u);
       for (uint32_t b = 8u * seg; b < bend; ++b) {
@@ -371,25 +450,38 @@ __device__ __forceinline__ void update_unit(const WaveMem &M, const LDS DevTable
   // 3. statep recurrences
   if (lane < 8) {
     LDS uint32_t *sp = M.stp + lane * 256;
-    const LDS uint8_t *S = M.rg + lane * Umax;
-    uint32_t s = S[0];
+    const LDS u32x4 *S4 = (const LDS u32x4 *)(M.rg + lane * Umax);
+    const uint32_t nblk = (nvalid + 15) >> 4;
+    u32x4 cur = S4[0];
+    uint32_t s = cur.x & 255u;
     uint32_t p = sp[s];
+    for (uint32_t blk = 0; blk < nblk; ++blk) {
+      const u32x4 nxt = (blk + 1 < nblk) ? S4[blk + 1] : cur;
+      const uint32_t seg = blk >> 2;
+      uint64_t yw = ym[0];
 #pragma unroll
-    for (int seg = 0; seg < 4; ++seg) {
-      const uint64_t yseg = ym[seg];
-      const uint32_t rend = min(nvalid, 64u * seg + 64u);
-      for (uint32_t r = 64u * seg; r < rend; ++r) {
-        const uint32_t sn = (r + 1 < nvalid) ? S[r + 1] : s;
-        const uint32_t pn_raw = sp[sn];
-        const uint32_t y = (uint32_t)(yseg >> (r & 63)) & 1u;
-        const uint32_t n = p & 1023u;
-        const int32_t e = (int32_t)((y << 22) - (p >> 10));
-        const uint32_t adj = ((uint32_t)(e >> 3) * adapt_rate(n)) & 0xfffffc00u;
-        const uint32_t pn = p + adj + (n < 1023u ? 1u : 0u);
-        sp[s] = pn;
-        p = (sn == s) ? pn : pn_raw;
-        s = sn;
+      for (int q = 1; q < NJ; ++q)
+        if (seg == (uint32_t)q) yw = ym[q];
+      yw >>= 16 * (blk & 3u);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const uint32_t r = blk * 16 + i;
+        if (r < nvalid) {
+          const uint32_t word = (i + 1) < 16 ? cur[(i + 1) >> 2] : nxt.x;
+          uint32_t sn = (word >> (8 * ((i + 1) & 3))) & 255u;
+          if (r + 1 >= nvalid) sn = s;
+          const uint32_t pn_raw = sp[sn];
+          const uint32_t y = (uint32_t)(yw >> i) & 1u;
+          const uint32_t n = p & 1023u;
+          const int32_t e = (int32_t)((y << 22) - (p >> 10));
+          const uint32_t adj = ((uint32_t)(e >> 3) * adapt_rate(n)) & 0xfffffc00u;
+          const uint32_t pn = p + adj + (n < 1023u ? 1u : 0u);
+          sp[s] = pn;
+          p = (sn == s) ? pn : pn_raw;
+          s = sn;
+        }
       }
+      cur = nxt;
     }
   }
   wave_sync();
@@ -421,6 +513,7 @@ __device__ __forceinline__ void update_unit(const WaveMem &M, const LDS DevTable
 // Encoder
 // ---------------------------------------------------------------------------
 
+template <int NJ>
 __device__ __forceinline__ void encode_task(const CodecArgs &a, const Task &tk, uint32_t t, const WaveMem &M,
                             const LDS DevTables *T, GLB uint8_t *slot, uint32_t Umax, uint32_t lane) {
   const uint32_t U = tk.U, tb = tk.tb, mask = (1u << tb) - 1u;
@@ -448,56 +541,111 @@ __device__ __forceinline__ void encode_task(const CodecArgs &a, const Task &tk, 
       }
       wave_sync();
       clk.mark(1);
-      // predict: lane handles unit bits r = lane + 64j; gathers of all its
-      // bits are in flight together
-      uint32_t sl[4][8], g[4][8];
+      // predict: lane handles unit bits r = lane + 64j; each layer of
+      // independent memory operations is issued before any result is used
+      uint32_t sl[NJ][8], g[NJ][8];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t r = lane + 64u * j;
-        if (r < nvalid) {
-          const uint32_t bi = r >> 3, jb = r & 7u;
-          const uint32_t cur = M.ub[8 + bi];
-          const uint32_t h0 = (1u << jb) | (jb ? (cur >> (8 - jb)) : 0u);
-          uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
-          sl[j][0] = h0;
+      for (int j = 0; j < NJ; ++j) {
+        // every lane computes (bits past nvalid read in-bounds unit bytes);
+        // only the stores below are predicated
+        const uint32_t r = min(lane + 64u * j, Umax - 1);
+        const uint32_t bi = r >> 3, jb = r & 7u;
+        const uint32_t cur = M.ub[8 + bi];
+        const uint32_t h0 = (1u << jb) | (jb ? (cur >> (8 - jb)) : 0u);
+        uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
+        sl[j][0] = h0;
 #pragma unroll
-          for (int k = 1; k < 8; ++k) {
-            H = (H ^ (uint32_t)M.ub[8 + bi - k]) * kFnvPrime;
-            sl[j][k] = H & mask;
-          }
-          g[j][0] = M.st0[h0];
-#pragma unroll
-          for (int k = 1; k < 8; ++k) g[j][k] = slot[((uint64_t)(k - 1) << tb) + sl[j][k]];
+        for (int k = 1; k < 8; ++k) {
+          H = (H ^ (uint32_t)M.ub[8 + bi - k]) * kFnvPrime;
+          sl[j][k] = H & mask;
         }
+        g[j][0] = M.st0[h0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) g[j][k] = slot[((uint64_t)(k - 1) << tb) + sl[j][k]];
       }
       if (clk.on) __builtin_amdgcn_s_waitcnt(0x0f70);  // attribute the gather wait to phase 2
       clk.mark(2);
+      uint32_t idx[NJ][8], pv[NJ][8];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t r = lane + 64u * j;
-        if (r < nvalid) {
-          uint32_t idx[8];
-          insert_slots(M.tkey, TS, sl[j], idx);
-          M.ridx[r] = (uint16_t)sl[j][0];
-          M.rg[r] = (uint8_t)g[j][0];
+      for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pv[j][k] = M.stp[k * 256 + g[j][k]];
+      {
+        // distinct-slot tables: first probes of every key go out together;
+        // lanes past the valid bits probe a dummy word; one uniform test
+        // guards the (rare) re-probe path
+        uint32_t old[NJ][8];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const bool ok = lane + 64u * j < nvalid;
 #pragma unroll
           for (int k = 1; k < 8; ++k) {
-            M.ridx[k * Umax + r] = (uint16_t)idx[k];
-            M.rg[k * Umax + r] = (uint8_t)g[j][k];
+            idx[j][k] = sl[j][k] & (TS - 1);
+            LDS uint32_t *at = ok ? &M.tkey[(k - 1) * TS + idx[j][k]] : &M.dummy[lane];
+            old[j][k] = atomicCAS((uint32_t *)at, kEmpty, sl[j][k]);
           }
-          int32_t smp[8];
-          uint32_t active = 0;
+        }
+        bool retry = false;
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            smp[k] = T->stretch[M.stp[k * 256 + g[j][k]] >> 20];
-            active += (smp[k] != 1);
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+          for (int k = 1; k < 8; ++k)
+            retry |= (lane + 64u * j < nvalid) && old[j][k] != kEmpty && old[j][k] != sl[j][k];
+        if (__ballot(retry)) {
+#pragma unroll
+          for (int j = 0; j < NJ; ++j)
+            if (lane + 64u * j < nvalid)
+#pragma unroll
+              for (int k = 1; k < 8; ++k)
+                while (old[j][k] != kEmpty && old[j][k] != sl[j][k]) {
+                  idx[j][k] = (idx[j][k] + 1) & (TS - 1);
+                  old[j][k] = atomicCAS((uint32_t *)&M.tkey[(k - 1) * TS + idx[j][k]], kEmpty,
+                                        sl[j][k]);
+                }
+        }
+      }
+      if (clk.on) __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+      clk.mark(9);
+      int32_t smp[NJ][8];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) smp[j][k] = T->stretch[pv[j][k] >> 20];
+      if (clk.on) __builtin_amdgcn_s_waitcnt(0xc07f);
+      clk.mark(10);
+      uint32_t cxt[NJ];
+      int32_t wv[NJ][8];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const uint32_t r = min(lane + 64u * j, Umax - 1);
+        uint32_t active = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) active += (smp[j][k] != 1);
+        cxt[j] = active + (uint32_t)(M.ub[8 + (r >> 3) - 1] >> 5) * 10u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) wv[j][k] = M.w[cxt[j] * 8 + k];
+      }
+      if (clk.on) __builtin_amdgcn_s_waitcnt(0xc07f);
+      clk.mark(11);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const uint32_t r = lane + 64u * j;
+        const bool ok = r < nvalid;
+        const uint32_t rr = min(r, Umax - 1);
+        const uint32_t pr = mix_weights(wv[j], smp[j], T->squash);
+        const uint32_t y = ((uint32_t)M.ub[8 + (rr >> 3)] >> (7 - (rr & 7u))) & 1u;
+        // bits past the data store nothing (their rprm keeps the stale value
+        // the tail padding codes)
+        if (ok) {
+          M.rprm[rr] = (uint16_t)((pr << 1) | y);
+          M.rcxt[rr] = (uint8_t)cxt[j];
+          M.ridx[rr] = (uint16_t)sl[j][0];
+          M.rg[rr] = (uint8_t)g[j][0];
+#pragma unroll
+          for (int k = 1; k < 8; ++k) {
+            M.ridx[k * Umax + rr] = (uint16_t)idx[j][k];
+            M.rg[k * Umax + rr] = (uint8_t)g[j][k];
           }
-          const uint32_t bi = r >> 3, jb = r & 7u;
-          const uint32_t cxt = active + (uint32_t)(M.ub[8 + bi - 1] >> 5) * 10u;
-          const uint32_t pr = mix_predict(M.w + cxt * 8, smp, T->squash);
-          const uint32_t y = ((uint32_t)M.ub[8 + bi] >> (7 - jb)) & 1u;
-          M.rprm[r] = (uint16_t)((pr << 1) | y);
-          M.rcxt[r] = (uint8_t)cxt;
         }
       }
       wave_sync();
@@ -505,12 +653,12 @@ __device__ __forceinline__ void encode_task(const CodecArgs &a, const Task &tk, 
     }
     // Arithmetic coder: a uniform (scalar) loop; (pr, y) of unit bit r is
     // read from lane r & 63 of the register that holds bits 64j..64j+63.
-    uint32_t pv[4];
+    uint32_t pv[NJ];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) pv[j] = (lane + 64u * j < Umax) ? M.rprm[lane + 64u * j] : 0u;
+    for (int j = 0; j < NJ; ++j) pv[j] = (lane + 64u * j < Umax) ? M.rprm[lane + 64u * j] : 0u;
     const uint32_t nc = (uint32_t)min((uint64_t)U, coded_end - base);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NJ; ++j) {
       const uint32_t lo = 64u * j;
       const uint32_t hi = min(nc, lo + 64u);
       for (uint32_t r = lo; r < hi; ++r) {
@@ -537,7 +685,7 @@ __device__ __forceinline__ void encode_task(const CodecArgs &a, const Task &tk, 
       }
     }
     clk.mark(4);
-    if (nvalid) update_unit(M, T, slot, tb, Umax, nvalid, lane, clk);
+    if (nvalid) update_unit<NJ>(M, T, slot, tb, Umax, nvalid, lane, clk);
     wave_sync();
   }
   acc |= (x1 >> 24) << (8 * (uint32_t)(opos & 3));
@@ -582,6 +730,7 @@ __device__ __forceinline__ uint32_t lane_value(const uint32_t v[4], uint32_t q) 
   return __builtin_amdgcn_readlane(v[3], q - 192);
 }
 
+template <int NJ>
 __device__ __forceinline__ void decode_task(const CodecArgs &a, const Task &tk, uint32_t t, const WaveMem &M,
                             const LDS DevTables *T, GLB uint8_t *slot, uint32_t Umax, uint32_t lane) {
   const uint32_t U = tk.U, tb = tk.tb, mask = (1u << tb) - 1u;
@@ -610,26 +759,45 @@ __device__ __forceinline__ void decode_task(const CodecArgs &a, const Task &tk, 
       }
       const uint32_t j0 = rb & 7u, bi = rb >> 3;
       const uint32_t known = j0 ? ((uint32_t)M.ub[8 + bi] >> (8 - j0)) : 0u;
-      // speculative predictions: path q fixes the next plen(q) bits to pbits(q)
-      for (uint32_t it = lane; it < 8 * P; it += 64) {
-        const uint32_t q = it >> 3, k = it & 7u;
-        const uint32_t plen_q = 31u - __clz(q + 1);
-        if (rb + plen_q >= nbits) continue;
-        const uint32_t pbits = q + 1u - (1u << plen_q);
-        const uint32_t h0 = (1u << (j0 + plen_q)) | (known << plen_q) | pbits;
-        uint32_t g, sl;
-        if (k == 0) {
-          g = M.st0[h0];
-          sl = h0;
-        } else {
-          uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
-          for (uint32_t i = 1; i <= k; ++i) H = (H ^ (uint32_t)M.ub[8 + bi - i]) * kFnvPrime;
-          sl = H & mask;
-          g = slot[((uint64_t)(k - 1) << tb) + sl];
+      // speculative predictions: path q fixes the next plen(q) bits to
+      // pbits(q); item it = 8q + k, two items per lane per pass
+      uint32_t hb[8];
+#pragma unroll
+      for (int i = 1; i < 8; ++i) hb[i] = M.ub[8 + bi - i];
+      for (uint32_t it0 = lane; it0 < 8 * P; it0 += 128) {
+        uint32_t gq[2], sq[2];
+        bool ok[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const uint32_t it = it0 + 64u * h;
+          const uint32_t q = it >> 3, k = it & 7u;
+          const uint32_t plen_q = 31u - __clz(q + 1);
+          ok[h] = it < 8 * P && rb + plen_q < nbits;
+          if (ok[h]) {
+            const uint32_t pbits = q + 1u - (1u << plen_q);
+            const uint32_t h0 = (1u << (j0 + plen_q)) | (known << plen_q) | pbits;
+            uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
+#pragma unroll
+            for (int i = 1; i < 8; ++i) {
+              const uint32_t Hn = (H ^ hb[i]) * kFnvPrime;
+              H = ((uint32_t)i <= k) ? Hn : H;
+            }
+            sq[h] = k ? (H & mask) : h0;
+            gq[h] = k ? (uint32_t)slot[((uint64_t)(k - 1) << tb) + sq[h]] : (uint32_t)M.st0[h0];
+          }
         }
-        M.pg[q * 8 + k] = (uint8_t)g;
-        M.pslot[q * 8 + k] = sl;
-        M.psmp[q * 8 + k] = T->stretch[M.stp[k * 256 + g] >> 20];
+        uint32_t pq[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (ok[h]) pq[h] = M.stp[((it0 + 64u * h) & 7u) * 256 + gq[h]];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (ok[h]) {
+            const uint32_t it = it0 + 64u * h;
+            M.psmp[it] = T->stretch[pq[h] >> 20];
+            M.pg[it] = (uint8_t)gq[h];
+            M.pslot[it] = sq[h];
+          }
       }
       wave_sync();
       clk.mark(9);
@@ -685,17 +853,33 @@ __device__ __forceinline__ void decode_task(const CodecArgs &a, const Task &tk, 
       if (lane == 0) M.ub[8 + bi] = (uint8_t)(M.ub[8 + bi] | (pre << (8 - j0 - nd)));
       clk.mark(10);
       // record the taken paths' slots and unit-start states
-      if (lane < 8 * nd) {
-        const uint32_t i = lane >> 3, k = lane & 7u, r = rb + i;
+      {
+        const bool rec = lane < 8 * nd;
+        const uint32_t i = lane >> 3, k = lane & 7u;
+        const uint32_t r = min(rb + i, Umax - 1);
         const uint32_t q = (uint32_t)(chosen >> (8 * i)) & 255u;
-        M.rg[k * Umax + r] = M.pg[q * 8 + k];
+        const uint32_t gq = M.pg[q * 8 + k];
         const uint32_t sl = M.pslot[q * 8 + k];
-        M.ridx[k * Umax + r] = (uint16_t)(k == 0 ? sl : insert_slot(M.tkey + (k - 1) * TS, TS, sl));
+        const bool ins = rec && k != 0;
+        uint32_t h = sl & (TS - 1);
+        LDS uint32_t *tk = M.tkey + (k ? (k - 1) * TS : 0);
+        uint32_t old = atomicCAS((uint32_t *)(ins ? &tk[h] : &M.dummy[lane]), kEmpty, sl);
+        if (__ballot(ins && old != kEmpty && old != sl)) {
+          if (ins)
+            while (old != kEmpty && old != sl) {
+              h = (h + 1) & (TS - 1);
+              old = atomicCAS((uint32_t *)&tk[h], kEmpty, sl);
+            }
+        }
+        if (rec) {
+          M.rg[k * Umax + r] = (uint8_t)gq;
+          M.ridx[k * Umax + r] = (uint16_t)(k == 0 ? sl : h);
+        }
       }
       wave_sync();
       clk.mark(11);
     }
-    update_unit(M, T, slot, tb, Umax, nbits, lane, clk);
+    update_unit<NJ>(M, T, slot, tb, Umax, nbits, lane, clk);
     const uint32_t nb = nbits >> 3;
     for (uint32_t i = lane; i < nb; i += 64) dst[(base >> 3) + i] = M.ub[8 + i];
     uint8_t hist = 0;
@@ -722,7 +906,7 @@ __device__ __forceinline__ void stage_tables(const DevTables *g, LDS DevTables *
   __syncthreads();
 }
 
-template <bool kDecode>
+template <bool kDecode, int NJ>
 __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a, uint32_t Umax) {
   extern __shared__ __align__(16) uint8_t lds[];
   LDS DevTables *T = (LDS DevTables *)lds;
@@ -741,9 +925,9 @@ __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a, uint32_t Umax) 
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
     if (kDecode)
-      decode_task(a, tk, t, M, T, slot, Umax, lane);
+      decode_task<NJ>(a, tk, t, M, T, slot, Umax, lane);
     else
-      encode_task(a, tk, t, M, T, slot, Umax, lane);
+      encode_task<NJ>(a, tk, t, M, T, slot, Umax, lane);
   }
 }
 
@@ -898,23 +1082,29 @@ uint32_t codec_lds_bytes(uint32_t Umax, uint32_t spec_bits, bool decode, uint32_
   return align16(sizeof(DevTables)) + waves * wave_lds_bytes(Umax, P, decode);
 }
 
+template <bool kDecode, int NJ>
+static hipError_t launch_one(const CodecArgs &a, uint32_t Umax, uint32_t blocks, uint32_t wpb,
+                             uint32_t lds, hipStream_t stream) {
+  hipError_t e = hipFuncSetAttribute((const void *)codec_kernel<kDecode, NJ>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((codec_kernel<kDecode, NJ>), dim3(blocks), dim3(64 * wpb), lds, stream, a,
+                     Umax);
+  return hipGetLastError();
+}
+
 hipError_t launch_codec(bool decode, const CodecArgs &a, uint32_t Umax, uint32_t blocks,
                         uint32_t waves_per_block, hipStream_t stream) {
   const uint32_t lds = codec_lds_bytes(Umax, a.spec_bits, decode, waves_per_block);
+  const int nj = Umax <= 64 ? 1 : (Umax <= 128 ? 2 : 4);
   if (decode) {
-    hipError_t e = hipFuncSetAttribute((const void *)codec_kernel<true>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(codec_kernel<true>, dim3(blocks), dim3(64 * waves_per_block), lds, stream,
-                       a, Umax);
-  } else {
-    hipError_t e = hipFuncSetAttribute((const void *)codec_kernel<false>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(codec_kernel<false>, dim3(blocks), dim3(64 * waves_per_block), lds, stream,
-                       a, Umax);
+    if (nj == 1) return launch_one<true, 1>(a, Umax, blocks, waves_per_block, lds, stream);
+    if (nj == 2) return launch_one<true, 2>(a, Umax, blocks, waves_per_block, lds, stream);
+    return launch_one<true, 4>(a, Umax, blocks, waves_per_block, lds, stream);
   }
-  return hipGetLastError();
+  if (nj == 1) return launch_one<false, 1>(a, Umax, blocks, waves_per_block, lds, stream);
+  if (nj == 2) return launch_one<false, 2>(a, Umax, blocks, waves_per_block, lds, stream);
+  return launch_one<false, 4>(a, Umax, blocks, waves_per_block, lds, stream);
 }
 
 hipError_t launch_read_headers(const uint64_t *src, const uint64_t *len, uint32_t n, uint8_t *hdr,
