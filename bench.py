@@ -210,6 +210,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-chunks", type=int, default=0)
+    ap.add_argument("--files", type=int, default=10000, help="batch workload: number of files")
+    ap.add_argument("--chunk-kib", type=int, default=256, help="batch workload: largest chunk")
+    ap.add_argument("--plain-files", type=int, default=40,
+                    help="batch workload: files in the one-launch-per-file sample")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the codec shards by chunk with no exchange on the "
