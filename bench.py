@@ -117,7 +117,7 @@ def stream_workload(args):
     achieved = ALG_BYTES_PER_INPUT_BYTE * n / (kms / 1000.0) / 1e9
     roofline = {"bound": "hbm", "kernel": "codec_kernel<%s>" % dom, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None}
+                "traffic": measured_traffic(dom, n)}
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -135,6 +135,17 @@ def stream_workload(args):
     line.update(res)
     line["oracle_sample_identical"] = parity
     return line
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of the dominant kernel: the fetched + written bytes
+    per input byte from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (calibrated against a plain copy kernel), times the workload size."""
+    path = os.path.join(ROOT, "profiles", "codec_kernel_traffic.json")
+    if not os.path.exists(path):
+        return None
+    t = json.load(open(path))[kernel]
+    return (t["fetch_bytes_per_input_byte"] + t["write_bytes_per_input_byte"]) * n
 
 
 def chunk_payloads(container, nchunks_max):
@@ -204,7 +215,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="stream", choices=["stream", "batch", "serve"])
+    ap.add_argument("--workload", default="stream",
+                    choices=["stream", "batch", "serve", "gather", "copy"])
     ap.add_argument("--mib", type=int, default=256)
     ap.add_argument("--spec", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -224,6 +236,12 @@ def main():
     elif args.workload == "batch":
         from cucm_amd.workloads import batch_workload
         line = batch_workload(args)
+    elif args.workload == "gather":
+        from cucm_amd.workloads import gather_workload
+        line = gather_workload(args)
+    elif args.workload == "copy":
+        from cucm_amd.workloads import copy_workload
+        line = copy_workload(args)
     else:
         from cucm_amd.workloads import serve_workload
         line = serve_workload(args)

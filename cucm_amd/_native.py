@@ -25,6 +25,7 @@ EXPORTS = [
     "cucm_ctx_synchronize", "cucm_ctx_last_stats", "cucm_ctx_set_max_waves",
     "cucm_ctx_set_phase_profile", "cucm_ctx_phase_cycles",
     "cucm_serve_start", "cucm_serve_stop", "cucm_submit", "cucm_poll",
+    "cucm_gather_probe", "cucm_copy_probe",
 ]
 
 
@@ -138,6 +139,8 @@ def lib():
         L.cucm_serve_stop.argtypes = [vp]
         L.cucm_submit.argtypes = [vp, P(Job), P(u64)]
         L.cucm_poll.argtypes = [vp, u64, ctypes.c_int, P(Result)]
+        L.cucm_gather_probe.argtypes = [vp, u64, u32, u32, u32, u32, u32, P(ctypes.c_double), P(u64)]
+        L.cucm_copy_probe.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         _lib = L
     return _lib
 

@@ -48,7 +48,32 @@ __global__ void gather_kernel(uint8_t *table, uint64_t bytes, uint32_t steps, ui
   if (acc == 0xdeadbeefu) sink[gid] = acc;
 }
 
+__global__ void copy_kernel(const uint4 *src, uint4 *dst, uint64_t n16) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+// Plain streaming copy of `bytes` (a multiple of 16): the reference kernel
+// whose FETCH_SIZE / WRITE_SIZE readings calibrate the HBM traffic counters.
+extern "C" int cucm_copy_probe(void *dst, const void *src, uint64_t bytes, double *ms) {
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, 0);
+  hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, 0, (const uint4 *)src, (uint4 *)dst,
+                     bytes / 16);
+  (void)hipEventRecord(e1, 0);
+  hipError_t err = hipEventSynchronize(e1);
+  float f = 0;
+  (void)hipEventElapsedTime(&f, e0, e1);
+  *ms = f;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return err == hipSuccess ? 0 : -6;
+}
 
 extern "C" int cucm_gather_probe(void *table, uint64_t bytes, uint32_t blocks, uint32_t threads,
                                  uint32_t steps, uint32_t width, uint32_t write_back,

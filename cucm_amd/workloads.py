@@ -114,3 +114,47 @@ def batch_workload(args):
 
 def serve_workload(args):
     raise SystemExit("the serving workload needs the serving loop (cucm_serve_start)")
+
+
+def gather_workload(args):
+    """Random one-byte read-modify-write touch rate over a footprint equal to
+    the stream workload's live model (resident waves x 7 order tables of
+    2^table_bits bytes): the ceiling for the codec's state gathers."""
+    import ctypes
+    from ._native import lib
+    tb = getattr(args, "gather_tb", 22)
+    waves = getattr(args, "gather_waves", 256)
+    footprint = waves * 7 * (1 << tb)
+    bytes_ = 1 << (footprint - 1).bit_length()
+    ctx = Context()
+    buf = ctx.alloc(bytes_)
+    res = {}
+    for name, blocks, thr, steps, width, wb in [("latency", 1, 64, 2000, 1, 0),
+                                                ("rmw_throughput", 4096, 256, 200, 8, 1),
+                                                ("read_throughput", 4096, 256, 200, 8, 0)]:
+        ms = ctypes.c_double()
+        clk = ctypes.c_uint64()
+        rc = lib().cucm_gather_probe(buf.addr, bytes_, blocks, thr, steps, width, wb,
+                                     ctypes.byref(ms), ctypes.byref(clk))
+        assert rc == 0
+        nw = blocks * thr // 64
+        res[name] = {"touches_per_s": blocks * thr * steps * width / (ms.value / 1e3),
+                     "clocks_per_step_per_wave": clk.value / nw / steps, "ms": ms.value}
+    return {"metric": "random one-byte touch rate", "footprint_bytes": bytes_,
+            "model_footprint_bytes": footprint, "results": res}
+
+
+def copy_workload(args):
+    """A plain 16-byte streaming copy of --mib MiB (HBM counter calibration)."""
+    import ctypes
+    from ._native import lib
+    n = args.mib * MiB
+    ctx = Context()
+    a = ctx.alloc(n)
+    b = ctx.alloc(n)
+    out = []
+    for _ in range(3):
+        ms = ctypes.c_double()
+        assert lib().cucm_copy_probe(b.addr, a.addr, n, ctypes.byref(ms)) == 0
+        out.append(ms.value)
+    return {"metric": "copy kernel", "bytes": n, "ms": out, "GBps": 2 * n / (min(out) / 1e3) / 1e9}

@@ -164,6 +164,19 @@ int cucm_ctx_phase_cycles(cucm_ctx *ctx, uint64_t cycles[CUCM_PHASES]);
 /* Upper bound on concurrently resident worker waves (0 = automatic). */
 int cucm_ctx_set_max_waves(cucm_ctx *ctx, uint32_t waves);
 
+/* ---- measurement probes (device 0 / current device) ------------------- */
+/* Random one-byte gather microbenchmark over `bytes` (a power of two) of
+ * device memory `table`: blocks x threads lanes each do `steps` rounds of
+ * `width` independent random reads (and, if write_back, writes of the same
+ * bytes).  The measured touch rate is the ceiling the codec's state gathers
+ * are compared against.  No counterpart in the reference. */
+int cucm_gather_probe(void *table, uint64_t bytes, uint32_t blocks, uint32_t threads,
+                      uint32_t steps, uint32_t width, uint32_t write_back, double *ms,
+                      uint64_t *wave_clocks);
+/* Streaming device copy of `bytes` (multiple of 16), used to calibrate the
+ * HBM traffic counters.  No counterpart in the reference. */
+int cucm_copy_probe(void *dst, const void *src, uint64_t bytes, double *ms);
+
 /* ---- serving: continuous batching ------------------------------------ */
 #define CUCM_OP_COMPRESS 0
 #define CUCM_OP_DECOMPRESS 1
