@@ -226,6 +226,8 @@ def main():
     ap.add_argument("--chunk-kib", type=int, default=256, help="batch workload: largest chunk")
     ap.add_argument("--plain-files", type=int, default=40,
                     help="batch workload: files in the one-launch-per-file sample")
+    ap.add_argument("--requests", type=int, default=240, help="serve workload: requests per run")
+    ap.add_argument("--lanes", type=int, default=0, help="serve workload: concurrent lanes (0 = 3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the codec shards by chunk with no exchange on the "

@@ -24,7 +24,7 @@ EXPORTS = [
     "cucm_ctx_malloc", "cucm_ctx_free", "cucm_ctx_memcpy_h2d", "cucm_ctx_memcpy_d2h",
     "cucm_ctx_synchronize", "cucm_ctx_last_stats", "cucm_ctx_set_max_waves",
     "cucm_ctx_set_phase_profile", "cucm_ctx_phase_cycles",
-    "cucm_serve_start", "cucm_serve_stop", "cucm_submit", "cucm_poll",
+    "cucm_serve_start", "cucm_serve_stop", "cucm_submit", "cucm_poll", "cucm_serve_launches",
     "cucm_gather_probe", "cucm_copy_probe",
 ]
 
@@ -83,7 +83,7 @@ class Job(ctypes.Structure):
 class Result(ctypes.Structure):
     _fields_ = [
         ("status", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("batch_jobs", ctypes.c_int32),
         ("out_len", ctypes.c_uint64),
         ("queue_ms", ctypes.c_double),
         ("latency_ms", ctypes.c_double),
@@ -135,7 +135,8 @@ def lib():
         L.cucm_ctx_set_max_waves.argtypes = [vp, u32]
         L.cucm_ctx_set_phase_profile.argtypes = [vp, ctypes.c_int]
         L.cucm_ctx_phase_cycles.argtypes = [vp, P(u64)]
-        L.cucm_serve_start.argtypes = [vp, u32]
+        L.cucm_serve_start.argtypes = [vp, u32, u32]
+        L.cucm_serve_launches.argtypes = [vp, P(u64)]
         L.cucm_serve_stop.argtypes = [vp]
         L.cucm_submit.argtypes = [vp, P(Job), P(u64)]
         L.cucm_poll.argtypes = [vp, u64, ctypes.c_int, P(Result)]

@@ -251,6 +251,53 @@ class Context:
                   "decompress_device")
         return list(olen), list(st)
 
+    # ---- serving loop (continuous batching) ------------------------------
+    def serve_start(self, lanes=0, max_inflight=0):
+        """Starts the request loop: `lanes` concurrent launches (0 = 3)."""
+        check(lib().cucm_serve_start(self.handle, lanes, max_inflight), "serve_start")
+        self._inflight = {}
+
+    def serve_stop(self):
+        """Waits for every submitted request, then stops the loop."""
+        check(lib().cucm_serve_stop(self.handle), "serve_stop")
+
+    def serve_launches(self) -> int:
+        v = ctypes.c_uint64()
+        check(lib().cucm_serve_launches(self.handle, ctypes.byref(v)), "serve_launches")
+        return v.value
+
+    def _submit(self, op, data, params, cap):
+        addr, n, keep = _buf(data)
+        out = np.empty(max(cap, 1), dtype=np.uint8)
+        job = Job(op=op, inp=addr, in_len=n, params=params, out=out.ctypes.data, out_cap=cap)
+        jid = ctypes.c_uint64()
+        check(lib().cucm_submit(self.handle, ctypes.byref(job), ctypes.byref(jid)), "submit")
+        self._inflight[jid.value] = (keep, out)
+        return jid.value
+
+    def submit_compress(self, data, params) -> int:
+        n = len(data)
+        cap = n + n // 2 + 4096 * (2 + n // max(params.chunk_size, 1))
+        return self._submit(0, data, params, cap)
+
+    def submit_decompress(self, blob, spec_bits=4) -> int:
+        size = int.from_bytes(bytes(blob[:4]), "big") if len(blob) >= 4 else 0
+        return self._submit(1, blob, make_params(spec_bits=spec_bits), size)
+
+    def poll(self, job_id, wait=True):
+        """Returns (output bytes, result dict) once the job is done, or
+        None while it runs (wait=False)."""
+        r = Result()
+        rc = lib().cucm_poll(self.handle, job_id, 1 if wait else 0, ctypes.byref(r))
+        if rc == -8:
+            return None
+        check(rc, "poll")
+        keep, out = self._inflight.pop(job_id)
+        info = {"status": r.status, "batch_jobs": r.batch_jobs, "queue_ms": r.queue_ms,
+                "latency_ms": r.latency_ms, "out_len": r.out_len}
+        check(r.status, "job %d" % job_id)
+        return out[: r.out_len].tobytes(), info
+
 
 _default = None
 

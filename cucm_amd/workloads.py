@@ -9,7 +9,8 @@ per batch, and is timed on a seeded random sample of the same files because
 each single-file launch runs alone on one wave.
 
 serve_workload: Poisson arrivals of mixed compress/decompress requests
-through the continuous-batching serving loop (cucm_serve_*).
+through the continuous-batching serving loop (cucm_serve_*), p50/p99 per
+load level beside the one-request-at-a-time plain path.
 """
 import os
 import sys
@@ -112,8 +113,128 @@ def batch_workload(args):
     return res
 
 
+def _pct(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def _fifo_latencies(arrivals, service):
+    """Latencies of a single FIFO server (one launch per request, one at a
+    time) for the given arrival times and measured service times (Lindley)."""
+    free_at = 0.0
+    lat = []
+    for a, s in zip(arrivals, service):
+        start = max(a, free_at)
+        free_at = start + s
+        lat.append(free_at - a)
+    return lat
+
+
 def serve_workload(args):
-    raise SystemExit("the serving workload needs the serving loop (cucm_serve_start)")
+    """Poisson arrivals of mixed requests through the serving loop.
+
+    Requests: 50% compress of fresh payloads, 50% decompress of containers
+    made in a warm-up; payload sizes min(round(256 KiB * exp(1.2 Z)), 128 MiB),
+    content from the order-2 Markov source (request i seeded 5000 + i); the
+    small-config model (16 KiB chunks, 2^18 slots, U = 128).  First the
+    saturated rate is measured by submitting every request at once; then
+    arrivals are replayed at lambda = f x saturated MB/s / mean size for each
+    load factor f.  The plain path (one launch per request, one request at a
+    time) is timed per request and its FIFO latencies computed for the same
+    arrival times.  Every output is compared with the same request run alone.
+    """
+    import threading
+    count = getattr(args, "requests", 240) or 240
+    S, tb = 16384, 18
+    params = make_params(S, 128, 100, 0, tb)
+    rng = np.random.default_rng(11)
+    sizes = datagen.lognormal_sizes(count, 262144, 1.2, 1, 128 * MiB, 13)
+    pool_sizes = datagen.lognormal_sizes(count // 2 + 1, 262144, 1.2, 1, 128 * MiB, 17)
+    arena, offs = datagen.markov2_files(sizes, seed0=5000)
+    parena, poffs = datagen.markov2_files(pool_sizes, seed0=9000)
+    fresh = [arena[int(o):int(o) + int(n)] for o, n in zip(offs, sizes)]
+    pool = [parena[int(o):int(o) + int(n)] for o, n in zip(poffs, pool_sizes)]
+    ctx = Context()
+    # warm-up: containers that the decompress requests replay
+    blobs = ctx.compress_many(pool, params)
+    ops = rng.random(count) < 0.5
+    reqs = []
+    for i in range(count):
+        if ops[i]:
+            reqs.append(("c", fresh[i], len(fresh[i])))
+        else:
+            k = int(rng.integers(len(pool)))
+            reqs.append(("d", (blobs[k], k), len(pool[k])))
+    mean_bytes = float(np.mean([r[2] for r in reqs]))
+    total_bytes = float(sum(r[2] for r in reqs))
+    # results of each request run alone, and the plain path's service times
+    alone, service = [], []
+    for kind, x, n in reqs:
+        t0 = time.perf_counter()
+        if kind == "c":
+            out = ctx.compress(x, params)
+        else:
+            out = ctx.decompress(x[0])
+        service.append(time.perf_counter() - t0)
+        alone.append(out)
+    plain_MBps = total_bytes / 1e6 / sum(service)
+    log("serve: %d requests, %.1f MB, plain path %.1f MB/s" % (count, total_bytes / 1e6, plain_MBps))
+
+    def run(arrivals):
+        ctx.serve_start(args.lanes, 0)
+        ids = [None] * count
+        results = [None] * count
+
+        def poller():
+            for i in range(count):
+                while ids[i] is None:
+                    time.sleep(0.0002)
+                results[i] = ctx.poll(ids[i], wait=True)
+
+        th = threading.Thread(target=poller)
+        th.start()
+        t0 = time.perf_counter()
+        for i, (kind, x, n) in enumerate(reqs):
+            if arrivals is not None:
+                d = t0 + arrivals[i] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            ids[i] = ctx.submit_compress(x, params) if kind == "c" else ctx.submit_decompress(x[0])
+        th.join()
+        makespan = time.perf_counter() - t0
+        launches = ctx.serve_launches()
+        ctx.serve_stop()
+        same = all(results[i][0] == alone[i] for i in range(count))
+        lat = [r[1]["latency_ms"] for r in results]
+        return {"MBps": total_bytes / 1e6 / makespan, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+                "launches": launches, "mean_batch": float(np.mean([r[1]["batch_jobs"] for r in results])),
+                "identical_to_alone": bool(same), "makespan_s": makespan}
+
+    sat = run(None)
+    log("serve saturated: %.1f MB/s, %d launches" % (sat["MBps"], sat["launches"]))
+    sweeps = []
+    for f in [0.25, 0.5, 0.75, 0.9, 1.0, 1.2]:
+        lam = f * sat["MBps"] * 1e6 / mean_bytes
+        arr = datagen.poisson_arrivals(count, lam, 19)
+        r = run(arr)
+        plain = [1000.0 * x for x in _fifo_latencies(arr, service)]
+        r.update({"load": f, "lambda_per_s": lam, "offered_MBps": lam * mean_bytes / 1e6,
+                  "plain_p50_ms": _pct(plain, 50), "plain_p99_ms": _pct(plain, 99)})
+        log("serve load %.2f: %.1f MB/s p50 %.1f ms p99 %.1f ms (plain p50 %.1f p99 %.1f)"
+            % (f, r["MBps"], r["p50_ms"], r["p99_ms"], r["plain_p50_ms"], r["plain_p99_ms"]))
+        sweeps.append(r)
+    ok = sat["identical_to_alone"] and all(r["identical_to_alone"] for r in sweeps)
+    return {
+        "metric": "compress/decompress MB/s on 1 GPU at identical ratio; batched vs plain; p99",
+        "value": sat["MBps"], "unit": "MB/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1000 * sat["makespan_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8/int32 fixed-point",
+        "data": "synthetic (seeded generator, host buffers, PCIe included)",
+        "config": {"workload": "serve: %d Poisson requests, 50%% compress / 50%% decompress, "
+                               "lognormal sizes (median 256 KiB, sigma 1.2), 16 KiB chunks, "
+                               "2^18 slots, U=128" % count, "lanes": args.lanes or 3},
+        "saturated": sat, "plain_MBps": plain_MBps, "plain_p50_ms": _pct([1000 * s for s in service], 50),
+        "sweep": sweeps, "identical_to_alone": bool(ok),
+    }
 
 
 def gather_workload(args):

@@ -178,30 +178,43 @@ int cucm_gather_probe(void *table, uint64_t bytes, uint32_t blocks, uint32_t thr
 int cucm_copy_probe(void *dst, const void *src, uint64_t bytes, double *ms);
 
 /* ---- serving: continuous batching ------------------------------------ */
+/* No counterpart in the reference (one file per process run there).
+ * Requests of mixed kinds and sizes are queued as they arrive; each of
+ * `lanes` worker lanes (own stream, device buffers and 1/lanes of the
+ * context's HBM budget) takes, whenever it falls idle, the oldest pending
+ * request plus every pending request of the same kind that fits its memory
+ * share, and runs them as one persistent launch.  Launches of different lanes
+ * overlap on the device.  A request's result never depends on its batch. */
 #define CUCM_OP_COMPRESS 0
 #define CUCM_OP_DECOMPRESS 1
 typedef struct cucm_job {
   int32_t op;              /* CUCM_OP_* */
   int32_t reserved;
-  const uint8_t *in;       /* host buffer */
+  const uint8_t *in;       /* host buffer, valid until cucm_poll reports the job */
   uint64_t in_len;
-  cucm_params params;      /* compress only */
+  cucm_params params;      /* compress: all fields; decompress: spec_bits only */
   uint8_t *out;            /* host buffer */
   uint64_t out_cap;
 } cucm_job;
 typedef struct cucm_result {
   int32_t status;          /* CUCM_OK or CUCM_E_* */
-  int32_t reserved;
+  int32_t batch_jobs;      /* requests that shared this request's launch */
   uint64_t out_len;
-  double queue_ms;         /* submit -> start on device */
-  double latency_ms;       /* submit -> result ready */
+  double queue_ms;         /* submit -> taken by a lane */
+  double latency_ms;       /* submit -> result in the caller's buffer */
 } cucm_result;
-/* Starts the serving loop on ctx (max_inflight jobs resident at once). */
-int cucm_serve_start(cucm_ctx *ctx, uint32_t max_inflight);
+/* Starts the serving loop on ctx: lanes (0 = 3, at most 16) concurrent
+ * launches, max_inflight (0 = 4096) requests taken by lanes at once. */
+int cucm_serve_start(cucm_ctx *ctx, uint32_t lanes, uint32_t max_inflight);
+/* Waits until every submitted request has completed, then stops the lanes.
+ * Must not run concurrently with cucm_submit/cucm_poll on the same ctx. */
 int cucm_serve_stop(cucm_ctx *ctx);
 int cucm_submit(cucm_ctx *ctx, const cucm_job *job, uint64_t *job_id);
-/* CUCM_E_NOTREADY while running; blocking waits if wait != 0. */
+/* CUCM_E_NOTREADY while running (wait == 0), else blocks until done.  A
+ * completed job is reported once; its id is then forgotten. */
 int cucm_poll(cucm_ctx *ctx, uint64_t job_id, int wait, cucm_result *res);
+/* Launches issued by the serving loop so far. */
+int cucm_serve_launches(cucm_ctx *ctx, uint64_t *launches);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,84 @@
+"""Serving loop (cucm_serve_start / cucm_submit / cucm_poll): requests of
+mixed kinds, sizes and parameters submitted together are batched into shared
+launches on concurrent lanes, and every result equals the same request run
+alone -- bit-exact, since the codec's output depends only on the request."""
+import numpy as np
+import pytest
+
+import cucm_amd
+from cucm_amd import datagen, make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    rng = np.random.default_rng(23)
+    reqs = []
+    for i in range(36):
+        n = int(rng.choice([0, 1, 7, 300, 5000, 40000, 150000, 400000]))
+        data = datagen.markov2(n, 300 + i).tobytes() if n else b""
+        kind = i % 4
+        if kind == 0:
+            p = make_params(16384, 128, 100, 0, 18)
+        elif kind == 1:
+            p = make_params(32768, 64, 3, 0, 16)
+        elif kind == 2:
+            p = make_params(8192, 256, 100, min(n, 2000), 14)
+        else:
+            p = make_params(65536, 96, 7, 0, 20, tail_pad=False)
+        reqs.append((data, p))
+    return reqs
+
+
+def test_serve_results_identical_alone_and_in_mix(ctx):
+    reqs = _requests()
+    alone_c = [ctx.compress(d, p) for d, p in reqs]
+    for (d, _), b in zip(reqs, alone_c):
+        assert ctx.decompress(b) == d
+    sc = cucm_amd.Context()
+    try:
+        sc.serve_start(lanes=3)
+        ids = []
+        for i, (d, p) in enumerate(reqs):
+            ids.append(("c", i, sc.submit_compress(d, p)))
+            # interleave decompressions of containers made alone
+            ids.append(("d", i, sc.submit_decompress(alone_c[i], spec_bits=[1, 2, 4, 8][i % 4])))
+        batch = []
+        for kind, i, jid in ids:
+            out, info = sc.poll(jid, wait=True)
+            batch.append(info["batch_jobs"])
+            if kind == "c":
+                assert out == alone_c[i], "compress request %d differs in the mix" % i
+            else:
+                assert out == reqs[i][0], "decompress request %d differs in the mix" % i
+        launches = sc.serve_launches()
+        sc.serve_stop()
+    finally:
+        sc.close()
+    assert launches < len(ids), "requests were never batched"
+    assert max(batch) > 1
+
+
+def test_serve_poll_nowait_and_errors(ctx):
+    sc = cucm_amd.Context()
+    try:
+        sc.serve_start(lanes=2, max_inflight=4)
+        d = datagen.markov2(100000, 77).tobytes()
+        jid = sc.submit_compress(d, make_params(16384, 128, 100, 0, 18))
+        got = None
+        while got is None:
+            got = sc.poll(jid, wait=False)
+        blob, info = got
+        assert info["latency_ms"] >= info["queue_ms"] >= 0
+        assert blob == ctx.compress(d, make_params(16384, 128, 100, 0, 18))
+        # a corrupt container fails alone, without hurting its neighbours
+        bad = bytearray(blob)
+        bad[25:29] = b"\xff\xff\xff\xff"
+        j_bad = sc.submit_decompress(bytes(bad))
+        j_ok = sc.submit_decompress(blob)
+        with pytest.raises(cucm_amd.CucmError):
+            sc.poll(j_bad, wait=True)
+        assert sc.poll(j_ok, wait=True)[0] == d
+        sc.serve_stop()
+    finally:
+        sc.close()
