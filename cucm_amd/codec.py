@@ -44,6 +44,17 @@ def compress_bound(n, params):
     return lib().cucm_compress_bound(n, ctypes.byref(params))
 
 
+def output_cap(n, params):
+    """A container buffer size that real inputs fit: 1.5x the chunk-padded
+    input (the last chunk is coded up to the full chunk size unless tail
+    padding is off), or the strict bound when that is smaller."""
+    S = max(params.chunk_size, 1)
+    chunks = (max(n - params.prelearn_size, 0) + S - 1) // S
+    padded = params.prelearn_size + chunks * S
+    cap = padded + padded // 2 + 4096 * (2 + chunks)
+    return min(cap, int(compress_bound(n, params)))
+
+
 def peek_header(blob):
     addr, n, keep = _buf(blob)
     orig = ctypes.c_uint64(0)
@@ -152,10 +163,8 @@ class Context:
         ins = (ctypes.c_void_p * n)(*[b[0] for b in bufs])
         lens = (ctypes.c_uint64 * n)(*[b[1] for b in bufs])
         ps = (Params * n)(*plist)
-        caps = [int(compress_bound(b[1], p)) for b, p in zip(bufs, plist)]
-        # realistic output fits 1.5x; the library reports E_SMALL otherwise
-        caps = [min(c, b[1] + b[1] // 2 + 4096 * (1 + b[1] // max(p.chunk_size, 1)))
-                for c, b, p in zip(caps, bufs, plist)]
+        # realistic output fits; the library reports E_SMALL otherwise
+        caps = [output_cap(b[1], p) for b, p in zip(bufs, plist)]
         outs = [np.empty(max(c, 1), dtype=np.uint8) for c in caps]
         outp = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
         capa = (ctypes.c_uint64 * n)(*caps)
@@ -276,9 +285,7 @@ class Context:
         return jid.value
 
     def submit_compress(self, data, params) -> int:
-        n = len(data)
-        cap = n + n // 2 + 4096 * (2 + n // max(params.chunk_size, 1))
-        return self._submit(0, data, params, cap)
+        return self._submit(0, data, params, output_cap(len(data), params))
 
     def submit_decompress(self, blob, spec_bits=4) -> int:
         size = int.from_bytes(bytes(blob[:4]), "big") if len(blob) >= 4 else 0
