@@ -1,0 +1,917 @@
+// cucm_wave.h -- the per-wave device code of the CuCM codec (model
+// primitives, unit update, encoder and decoder of one chunk), shared by the
+// gfx950 kernels (cucm_kernels.hip) and the CPU wave emulator of the tests.
+//
+// Execution model: a persistent grid of waves.  Each wave (64 lanes) owns
+// one model slot in HBM and pulls chunk tasks from a device work queue
+// (atomic counter), so thousands of uneven chunks or files run in a single
+// launch.  A chunk is processed one unit (U bits, the reference's frozen-model
+// "vector") at a time:
+//
+//   predict  all U bits x 8 orders in parallel across lanes: FNV context
+//            hash, one-byte state gather from the order's table in HBM
+//            (order 0 lives in LDS, indexed by the partial byte), statep and
+//            stretch lookups in LDS, 8-input integer mixer, squash.
+//   code     a uniform scalar loop runs the carry-less 32-bit binary
+//            arithmetic coder.
+//   update   mixer deltas by LDS integer atomics (order independent, exact);
+//            per-order StateMap recurrences walked in bit order by 8 lanes,
+//            with in-unit slot aliasing resolved through a per-order LDS
+//            open-addressing table of the unit's distinct slots; final states
+//            of the distinct slots written back to HBM once per unit.
+//
+// All arithmetic is the reference's fixed-point integer arithmetic
+// (cucm_compress_v2.cu:558-705, 711-868; cucm_decompress_v2.cu:698-896),
+// so encoder and decoder mirror each other bit for bit.
+//
+// Cross-lane primitives go through the small wrappers below; with
+// CUCM_WAVE_EMULATION defined, the including file supplies CPU versions of
+// them instead (64 lanes run as fibers that meet at every cross-lane
+// operation), so the same source can be checked on the host.
+#ifndef CUCM_WAVE_H
+#define CUCM_WAVE_H
+#include "cucm_model.h"
+
+#ifndef CUCM_WAVE_EMULATION
+#include <hip/hip_runtime.h>
+
+// Pointers into LDS carry the LDS address space so every access is a ds_* op.
+#define LDS __attribute__((address_space(3)))
+#define GLB __attribute__((address_space(1)))
+#define CUCM_DEV __device__ __forceinline__
+#define CUCM_HD __host__ __device__ inline
+#define CUCM_HDI __host__ __device__ __forceinline__
+
+namespace cucm {
+CUCM_DEV uint32_t lane_read(uint32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v, lane); }
+CUCM_DEV uint32_t lane_first(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+CUCM_DEV uint64_t wave_ballot(bool p) { return __ballot(p); }
+CUCM_DEV uint32_t clz32(uint32_t x) { return (uint32_t)__clz((int)x); }  // 32 for 0
+CUCM_DEV uint64_t shader_clock() { return clock64(); }
+CUCM_DEV void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0f70); }    // vmcnt(0)
+CUCM_DEV void wait_lgkm() { __builtin_amdgcn_s_waitcnt(0xc07f); }  // lgkmcnt(0)
+// Orders the wave's LDS traffic across lanes (and stops the compiler from
+// moving LDS accesses across it); lanes of a wave run in lock step.
+CUCM_DEV void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Waits for this wave's outstanding vector memory operations: a wave's state
+// write-backs must land before its own next gathers.  No cache maintenance
+// is needed for a wave's own accesses.
+CUCM_DEV void vm_drain() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  wait_vm();
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+CUCM_DEV uint32_t lds_cas(LDS uint32_t *p, uint32_t cmp, uint32_t v) {
+  return atomicCAS((uint32_t *)p, cmp, v);
+}
+CUCM_DEV void lds_add(LDS int32_t *p, int32_t v) { atomicAdd((int32_t *)p, v); }
+CUCM_DEV void glb_add64(uint64_t *p, uint64_t v) {
+  atomicAdd((unsigned long long *)p, (unsigned long long)v);
+}
+}  // namespace cucm
+#endif
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+namespace cucm {
+
+constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr uint16_t kNoState = 0x100;
+
+CUCM_HD uint32_t table_size(uint32_t U) {
+  uint32_t t = 64;
+  while (t < 2 * U) t <<= 1;
+  return t;
+}
+
+CUCM_HD uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
+
+// Per-wave LDS carve-out.  Laid out for the launch's largest unit (Umax)
+// and, for the decoder, the speculation width (P paths).
+struct WaveMem {
+  LDS uint32_t *stp;    // [8][256] statep
+  LDS int32_t *w;       // [80][8] mixer weights
+  LDS uint32_t *tkey;   // [7][TS] distinct slots of the unit, orders 1..7
+  LDS uint32_t *pslot;  // [2][P][8] decoder: slot (order 0: h0) per path, two rounds
+  LDS uint16_t *tval;   // [7][TS] running state of each distinct slot
+  LDS uint16_t *ridx;   // [8][Umax] table index (order 0: h0) per unit bit
+  LDS uint16_t *rprm;   // [Umax] pr << 1 | y per unit bit (persists: tail padding)
+  LDS uint8_t *st0;     // [256] order-0 states by h0
+  LDS uint8_t *rg;      // [8][Umax] unit-start state per unit bit
+  LDS uint8_t *rcxt;    // [Umax] mixer context per unit bit
+  LDS uint8_t *pg;      // [2][P][8] decoder: unit-start state per path, two rounds
+  LDS uint8_t *ub;      // [8 + Umax/8] 8 history bytes + the unit's bytes
+  LDS uint32_t *dummy;  // [64] scratch target of predicated-off LDS operations
+};
+
+// Byte offsets of the per-wave LDS arrays (16-byte aligned each).
+struct WaveLayout {
+  uint32_t stp, w, tkey, pslot, tval, ridx, rprm, st0, rg, rcxt, pg, ub, dummy, total;
+};
+
+CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec) {
+  const uint32_t TS = table_size(Umax);
+  const uint32_t Pd = dec ? P : 0u;
+  WaveLayout l;
+  uint32_t off = 0;
+#define CUCM_TAKE(field, bytes) \
+  l.field = off;                \
+  off = align16(off + (bytes));
+  CUCM_TAKE(stp, 8 * 256 * 4)
+  CUCM_TAKE(w, kMixCtx * kOrders * 4)
+  CUCM_TAKE(tkey, 7 * TS * 4)
+  CUCM_TAKE(pslot, 2 * Pd * 8 * 4)
+  CUCM_TAKE(tval, 7 * TS * 2)
+  CUCM_TAKE(ridx, 8 * Umax * 2)
+  CUCM_TAKE(rprm, Umax * 2)
+  CUCM_TAKE(st0, 256)
+  CUCM_TAKE(rg, 8 * Umax)
+  CUCM_TAKE(rcxt, Umax)
+  CUCM_TAKE(pg, 2 * Pd * 8)
+  CUCM_TAKE(ub, 8 + Umax / 8 + 8)
+  CUCM_TAKE(dummy, 64 * 4)
+#undef CUCM_TAKE
+  l.total = off;
+  return l;
+}
+
+CUCM_HD uint32_t wave_lds_bytes(uint32_t Umax, uint32_t P, bool dec) {
+  return wave_layout(Umax, P, dec).total;
+}
+
+CUCM_DEV WaveMem carve(LDS uint8_t *base, uint32_t Umax, uint32_t P, bool dec) {
+  const WaveLayout l = wave_layout(Umax, P, dec);
+  WaveMem m;
+  m.stp = (LDS uint32_t *)(base + l.stp);
+  m.w = (LDS int32_t *)(base + l.w);
+  m.tkey = (LDS uint32_t *)(base + l.tkey);
+  m.pslot = (LDS uint32_t *)(base + l.pslot);
+  m.tval = (LDS uint16_t *)(base + l.tval);
+  m.ridx = (LDS uint16_t *)(base + l.ridx);
+  m.rprm = (LDS uint16_t *)(base + l.rprm);
+  m.st0 = base + l.st0;
+  m.rg = base + l.rg;
+  m.rcxt = base + l.rcxt;
+  m.pg = base + l.pg;
+  m.ub = base + l.ub;
+  m.dummy = (LDS uint32_t *)(base + l.dummy);
+  return m;
+}
+
+// Per-phase shader-clock totals of one wave (enabled when args.prof != null):
+//   0 model init   1 unit staging   2 hash + gather   3 slot tables + mix
+//   4 arithmetic coder   5 mixer deltas + collision scan   6 state resolution
+//   7 statep recurrences   8 write-back   9 decoder path predictions
+//   10 decoder mix + decode   11 decoder records
+struct PhaseClock {
+  bool on;
+  uint64_t last;
+  uint64_t acc[kPhases];
+  CUCM_DEV explicit PhaseClock(bool enable) : on(enable), last(0) {
+#pragma unroll
+    for (int i = 0; i < kPhases; ++i) acc[i] = 0;
+    if (on) last = shader_clock();
+  }
+  CUCM_DEV void mark(int ph) {
+    if (on) {
+      const uint64_t t = shader_clock();
+      acc[ph] += t - last;
+      last = t;
+    }
+  }
+  CUCM_DEV void flush(uint64_t *out, uint32_t lane) {
+    if (on && lane == 0)
+#pragma unroll
+      for (int i = 0; i < kPhases; ++i) glb_add64(&out[i], acc[i]);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Model primitives (device)
+// ---------------------------------------------------------------------------
+
+// Mixer::p (cucm_compress_v2.cu:636-644) plus the pr bump (:787-788).
+CUCM_DEV uint32_t mix_predict(const LDS int32_t *wrow, const int32_t smp[8],
+                                                const LDS uint16_t *squash) {
+  int64_t dot = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dot += (int64_t)wrow[k] * (int64_t)smp[k];
+  dot >>= 16;
+  const int idx = dot < -2047 ? -2047 : (dot > 2048 ? 2048 : (int)dot);
+  uint32_t pr = squash[idx + 2047];
+  if (pr < 2048) ++pr;
+  return pr;
+}
+
+// Mixer::p with the weight row already in registers.
+CUCM_DEV uint32_t mix_weights(const int32_t w[8], const int32_t smp[8],
+                                                const LDS uint16_t *squash) {
+  int64_t dot = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dot += (int64_t)w[k] * (int64_t)smp[k];
+  dot >>= 16;
+  const int idx = dot < -2047 ? -2047 : (dot > 2048 ? 2048 : (int)dot);
+  uint32_t pr = squash[idx + 2047];
+  if (pr < 2048) ++pr;
+  return pr;
+}
+
+// Carry-less coder interval split (cucm_compress_v2.cu:855-857):
+// x1 + (rng >> 12) * pr + (((rng & 0xfff) * pr) >> 12) with rng = x2 - x1.
+// Writing rng = 4096a + b, that sum is a*pr + floor(b*pr / 4096) =
+// floor(rng * pr / 4096), one 32x32->64 product (rng * pr < 2^44).
+CUCM_DEV uint32_t coder_mid(uint32_t x1, uint32_t x2, uint32_t pr) {
+  return x1 + (uint32_t)(((uint64_t)(x2 - x1) * pr) >> 12);
+}
+
+// Zeroes (or copies the prelearned model into) the wave's model slot and
+// resets the LDS-resident parts of the model.
+CUCM_DEV void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M, GLB uint8_t *slot,
+                           uint32_t Umax, uint32_t lane) {
+  const uint64_t bytes = 7ull << tk.tb;
+  const uint64_t n16 = bytes >> 4;
+  GLB u32x4 *dst = (GLB u32x4 *)slot;
+  if (tk.flags & kTaskInitModel) {
+    const DevInitModel *im = a.init + tk.model;
+    const GLB u32x4 *src = (const GLB u32x4 *)im->states;
+    for (uint64_t i = lane; i < n16; i += 64) dst[i] = src[i];
+    for (uint32_t i = lane; i < 8 * 256; i += 64) M.stp[i] = im->statep[i >> 8][i & 255];
+    for (uint32_t i = lane; i < kMixCtx * kOrders; i += 64) M.w[i] = im->w[i];
+    for (uint32_t i = lane; i < 256; i += 64) M.st0[i] = im->st0[i];
+  } else {
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    for (uint64_t i = lane; i < n16; i += 64) dst[i] = z;
+    for (uint32_t i = lane; i < 8 * 256; i += 64) M.stp[i] = 1u << 31;
+    for (uint32_t i = lane; i < kMixCtx * kOrders; i += 64) M.w[i] = 0;
+    for (uint32_t i = lane; i < 256; i += 64) M.st0[i] = 0;
+  }
+  const uint32_t TS = table_size(Umax);
+  for (uint32_t i = lane; i < 7 * TS; i += 64) {
+    M.tkey[i] = kEmpty;
+    M.tval[i] = kNoState;
+  }
+  for (uint32_t i = lane; i < Umax; i += 64) M.rprm[i] = 0;
+  for (uint32_t i = lane; i < 8 + Umax / 8 + 8; i += 64) M.ub[i] = 0;
+  vm_drain();
+  wave_sync();
+}
+
+// Applies the deferred updates of one unit (cucm_compress_v2.cu:794-839,
+// cucm_decompress_v2.cu:870-896), for unit bits r < nvalid (a multiple of 8):
+//   1. mixer deltas from the recorded (pr, y, cxt) and unit-start predictions
+//      (LDS integer atomics: the sum is order independent);
+//   2. state resolution: the state each StateMap update sees, s = state of
+//      the slot after all earlier bits of the unit.  Bits of one byte never
+//      share an order-0 slot (distinct partial bytes), so a byte's 8 bits x
+//      8 orders go in one step over all 64 lanes; a byte where two bits of an
+//      order hash to one slot (a hash collision) is stepped bit by bit;
+//   3. per order, the statep recurrence in bit order (8 lanes), with the next
+//      bit's statep read issued before the current write and forwarded when
+//      both hit the same state;
+//   4. the final state of every touched slot back to HBM.
+template <int NJ>
+CUCM_DEV void update_unit(const WaveMem &M, const LDS DevTables *T, GLB uint8_t *slot, uint32_t tb,
+                            uint32_t Umax, uint32_t nvalid, uint32_t lane, PhaseClock &clk) {
+  const uint32_t TS = table_size(Umax);
+  const uint32_t nbytes = nvalid >> 3;
+  // y of every unit bit as a uniform bit mask
+  uint64_t ym[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const uint32_t r = lane + 64u * j;
+    const bool y = (r < nvalid) && (M.rprm[r < Umax ? r : 0] & 1u);
+    ym[j] = wave_ballot(y);
+  }
+  // 1. mixer deltas
+  for (uint32_t r = lane; r < nvalid; r += 64) {
+    const uint32_t prm = M.rprm[r];
+    const int32_t err = ((int32_t)((prm & 1u) << 12) - (int32_t)(prm >> 1)) * 7;
+    const uint32_t cxt = M.rcxt[r];
+    int32_t d[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t g = M.rg[k * Umax + r];
+      const int32_t smp = T->stretch[M.stp[k * 256 + g] >> 20];
+      d[k] = (int32_t)(((int64_t)err * smp) >> 16);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lds_add(&M.w[cxt * 8 + k], d[k]);
+  }
+  // bytes with an in-byte slot collision in some order >= 1
+  uint32_t cmask = 0;
+  for (uint32_t base = 0; base < 7 * nbytes; base += 64) {
+    const uint32_t it = base + lane;
+    bool coll = false;
+    if (it < 7 * nbytes) {
+      const uint32_t b = it / 7, k = 1 + it % 7;
+      const u32x4 v = *(const LDS u32x4 *)(M.ridx + k * Umax + 8 * b);
+      const uint32_t e[8] = {v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16,
+                             v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16};
+#pragma unroll
+      for (int x = 0; x < 8; ++x)
+#pragma unroll
+        for (int z = x + 1; z < 8; ++z) coll |= (e[x] == e[z]);
+    }
+    uint64_t bal = wave_ballot(coll);
+    while (bal) {
+      const uint32_t bit = (uint32_t)__builtin_ctzll(bal);
+      bal &= bal - 1;
+      cmask |= 1u << ((base + bit) / 7);
+    }
+  }
+  wave_sync();
+  clk.mark(5);
+  // 2. state resolution, one byte of the unit per step
+  {
+    const uint32_t k = lane >> 3, j = lane & 7u;
+    LDS uint16_t *tv = M.tval + (k ? (k - 1) * TS : 0);
+#pragma unroll
+    for (int seg = 0; seg < NJ; ++seg) {
+      const uint64_t yseg = ym[seg];
+      const uint32_t bend = min(nbytes, 8u * seg + 8u);
+      for (uint32_t b = 8u * seg; b < bend; ++b) {
+        const uint32_t r = 8 * b + j;
+        const uint32_t y = (uint32_t)(yseg >> (r & 63)) & 1u;
+        const uint32_t i = M.ridx[k * Umax + r];
+        const uint32_t g = M.rg[k * Umax + r];
+        const uint32_t reps = ((cmask >> b) & 1u) ? 8u : 1u;
+        for (uint32_t q = 0; q < reps; ++q) {
+          if (reps == 1u || q == j) {
+            uint32_t st;
+            if (k == 0) {
+              st = M.st0[i];
+            } else {
+              const uint32_t v = tv[i];
+              st = v > 255u ? g : v;
+            }
+            const uint32_t ns = T->nex[2 * st + y];
+            if (k == 0)
+              M.st0[i] = (uint8_t)ns;
+            else
+              tv[i] = (uint16_t)ns;
+            M.rg[k * Umax + r] = (uint8_t)st;
+          }
+          wave_sync();
+        }
+      }
+    }
+  }
+  clk.mark(6);
+  // 3. statep recurrences: lane k < 8 walks order k's updates in bit order.
+  // Software-pipelined so that the dependent chain per bit is the update
+  // arithmetic alone: the statep entry of bit i+2 is read before bit i's
+  // write (the two latest writes are forwarded instead), and the adaptation
+  // rate dt[n] of every candidate is looked up a bit early -- the count n in
+  // an entry's low 10 bits advances to min(n + 1, 1023) on every update,
+  // whatever the bit, so it never waits for the arithmetic.
+  if (lane < 8) {
+    LDS uint32_t *sp = M.stp + lane * 256;
+    const LDS uint16_t *dtab = T->dt;
+    const LDS u32x4 *S4 = (const LDS u32x4 *)(M.rg + lane * Umax);
+    const uint32_t nblk = (nvalid + 15) >> 4;
+    u32x4 cur = S4[0];
+    u32x4 nxt = nblk > 1 ? S4[1] : cur;
+    uint32_t s0 = cur.x & 255u, s1 = (cur.x >> 8) & 255u, sprev = 0x100u;
+    uint32_t p = sp[s0];
+    uint32_t d = dtab[p & 1023u];
+    uint32_t raw1 = sp[s1];  // read before any write of this unit
+    uint32_t pnprev = 0, dfprev = 0;
+    for (uint32_t blk = 0; blk < nblk; ++blk) {
+      const uint32_t seg = blk >> 2;
+      uint64_t yw = ym[0];
+#pragma unroll
+      for (int q = 1; q < NJ; ++q)
+        if (seg == (uint32_t)q) yw = ym[q];
+      yw >>= 16 * (blk & 3u);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        if (blk * 16 + t < nvalid) {
+          const int t2 = t + 2;
+          const uint32_t w2 = t2 < 16 ? cur[t2 >> 2] : nxt[(t2 - 16) >> 2];
+          const uint32_t s2 = (w2 >> (8 * (t2 & 3))) & 255u;
+          const uint32_t raw2 = sp[s2];  // misses the writes of bits i, i+1
+          const uint32_t n = p & 1023u;
+          const uint32_t nn = min(n + 1u, 1023u);
+          const uint32_t dfwd = dtab[nn];
+          const uint32_t draw = dtab[raw1 & 1023u];
+          const uint32_t y22 = (uint32_t)((yw >> t) & 1u) << 22;
+          const int32_t e = (int32_t)(y22 - (p >> 10));
+          const uint32_t pn = p + (((uint32_t)(e >> 3) * d) & 0xfffffc00u) + (nn - n);
+          sp[s0] = pn;
+          const bool same1 = s1 == s0, same2 = s1 == sprev;
+          const uint32_t pnext = same1 ? pn : (same2 ? pnprev : raw1);
+          const uint32_t dnext = same1 ? dfwd : (same2 ? dfprev : draw);
+          sprev = s0;
+          s0 = s1;
+          s1 = s2;
+          pnprev = pn;
+          dfprev = dfwd;
+          raw1 = raw2;
+          p = pnext;
+          d = dnext;
+        }
+      }
+      cur = nxt;
+      nxt = (blk + 2 < nblk) ? S4[blk + 2] : cur;
+    }
+  }
+  wave_sync();
+  clk.mark(7);
+  // 4. write back the touched slots (every occurrence writes the same final
+  // state), then clear the tables
+#pragma unroll 1
+  for (uint32_t k = 1; k < 8; ++k) {
+    GLB uint8_t *tab = slot + ((uint64_t)(k - 1) << tb);
+    for (uint32_t r = lane; r < nvalid; r += 64) {
+      const uint32_t i = M.ridx[k * Umax + r];
+      tab[M.tkey[(k - 1) * TS + i]] = (uint8_t)M.tval[(k - 1) * TS + i];
+    }
+  }
+  wave_sync();
+#pragma unroll 1
+  for (uint32_t k = 1; k < 8; ++k)
+    for (uint32_t r = lane; r < nvalid; r += 64) {
+      const uint32_t i = M.ridx[k * Umax + r];
+      M.tkey[(k - 1) * TS + i] = kEmpty;
+      M.tval[(k - 1) * TS + i] = kNoState;
+    }
+  vm_drain();
+  wave_sync();
+  clk.mark(8);
+}
+
+// ---------------------------------------------------------------------------
+// Encoder
+// ---------------------------------------------------------------------------
+
+// Stages unit bytes, hashes every bit's 8 slots, issues the 7 state gathers
+// per bit and places every slot in the unit's distinct-slot tables (the
+// inserts run while the gathers are in flight).  Lane handles unit bits
+// r = lane + 64j; bits past nvalid compute in-bounds garbage and insert
+// nothing.
+template <int NJ>
+CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
+                                             uint64_t base, uint32_t U, uint32_t nvalid,
+                                             const GLB uint8_t *slot, uint32_t tb, uint32_t mask,
+                                             uint32_t Umax, uint32_t lane, uint32_t sl[NJ][8],
+                                             uint32_t g[NJ][8], uint32_t idx[NJ][8]) {
+  const uint32_t TS = table_size(Umax);
+  // 8 history bytes and the unit's bytes (zero outside the chunk)
+  const int64_t b0 = (int64_t)(base >> 3) - 8;
+  for (uint32_t i = lane; i < 8 + U / 8; i += 64) {
+    const int64_t p = b0 + i;
+    M.ub[i] = (p >= 0 && p < (int64_t)L) ? src[p] : (uint8_t)0;
+  }
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const uint32_t r = min(lane + 64u * j, Umax - 1);
+    const uint32_t bi = r >> 3, jb = r & 7u;
+    const uint32_t cur = M.ub[8 + bi];
+    const uint32_t h0 = (1u << jb) | (jb ? (cur >> (8 - jb)) : 0u);
+    uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
+    sl[j][0] = h0;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      H = (H ^ (uint32_t)M.ub[8 + bi - k]) * kFnvPrime;
+      sl[j][k] = H & mask;
+    }
+    g[j][0] = M.st0[h0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) g[j][k] = slot[((uint64_t)(k - 1) << tb) + sl[j][k]];
+  }
+  // distinct-slot tables: the first probes of every key go out together;
+  // lanes past the valid bits probe a dummy word; keys that meet another
+  // key probe on in rounds in which every unplaced key moves together
+  uint32_t old[NJ][8];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const bool ok = lane + 64u * j < nvalid;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      idx[j][k] = sl[j][k] & (TS - 1);
+      LDS uint32_t *at = ok ? &M.tkey[(k - 1) * TS + idx[j][k]] : &M.dummy[lane];
+      old[j][k] = lds_cas(at, kEmpty, sl[j][k]);
+    }
+  }
+  uint32_t pend = 0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+      if ((lane + 64u * j < nvalid) && old[j][k] != kEmpty && old[j][k] != sl[j][k])
+        pend |= 1u << (j * 8 + k);
+  while (wave_ballot(pend != 0)) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        const bool pb = (pend >> (j * 8 + k)) & 1u;
+        idx[j][k] = pb ? ((idx[j][k] + 1) & (TS - 1)) : idx[j][k];
+        LDS uint32_t *at = pb ? &M.tkey[(k - 1) * TS + idx[j][k]] : &M.dummy[lane];
+        old[j][k] = lds_cas(at, kEmpty, sl[j][k]);
+      }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int k = 1; k < 8; ++k)
+        if (old[j][k] == kEmpty || old[j][k] == sl[j][k]) pend &= ~(1u << (j * 8 + k));
+  }
+}
+
+// Completes the unit's predictions once its gathers are back: statep,
+// stretch, mixer context, mix (cucm_compress_v2.cu:711-792), and records
+// (pr, y), cxt, slots and unit-start states per bit for the coder and the
+// update.
+template <int NJ>
+CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
+                                               uint32_t nvalid, uint32_t Umax, uint32_t lane,
+                                               const uint32_t sl[NJ][8], const uint32_t g[NJ][8],
+                                               const uint32_t idx[NJ][8]) {
+  uint32_t pv[NJ][8];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pv[j][k] = M.stp[k * 256 + g[j][k]];
+  int32_t smp[NJ][8];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) smp[j][k] = T->stretch[pv[j][k] >> 20];
+  uint32_t cxt[NJ];
+  int32_t wv[NJ][8];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const uint32_t r = min(lane + 64u * j, Umax - 1);
+    uint32_t active = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) active += (smp[j][k] != 1);
+    cxt[j] = active + (uint32_t)(M.ub[8 + (r >> 3) - 1] >> 5) * 10u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) wv[j][k] = M.w[cxt[j] * 8 + k];
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const uint32_t r = lane + 64u * j;
+    const bool ok = r < nvalid;
+    const uint32_t rr = min(r, Umax - 1);
+    const uint32_t pr = mix_weights(wv[j], smp[j], T->squash);
+    const uint32_t y = ((uint32_t)M.ub[8 + (rr >> 3)] >> (7 - (rr & 7u))) & 1u;
+    // bits past the data store nothing (their rprm keeps the stale value
+    // the tail padding codes)
+    if (ok) {
+      M.rprm[rr] = (uint16_t)((pr << 1) | y);
+      M.rcxt[rr] = (uint8_t)cxt[j];
+      M.ridx[rr] = (uint16_t)sl[j][0];
+      M.rg[rr] = (uint8_t)g[j][0];
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        M.ridx[k * Umax + rr] = (uint16_t)idx[j][k];
+        M.rg[k * Umax + rr] = (uint8_t)g[j][k];
+      }
+    }
+  }
+  wave_sync();
+}
+
+// One chunk.  Per unit: stage, hash and gather, predictions, the arithmetic
+// coder, then the deferred update.
+template <int NJ>
+CUCM_DEV void encode_task(const CodecArgs &a, const Task &tk, uint32_t t,
+                                            const WaveMem &M, const LDS DevTables *T,
+                                            GLB uint8_t *slot, uint32_t Umax, uint32_t lane) {
+  const uint32_t U = tk.U, tb = tk.tb, mask = (1u << tb) - 1u;
+  const uint64_t L = tk.L;
+  const GLB uint8_t *src = (const GLB uint8_t *)(uintptr_t)tk.src;
+  GLB uint8_t *dst = (GLB uint8_t *)(uintptr_t)tk.dst;
+  const uint64_t cap = tk.aux;
+  PhaseClock clk(a.prof != nullptr);
+  init_model(a, tk, M, slot, Umax, lane);
+  clk.mark(0);
+  const uint64_t vbits = 8 * L;
+  const uint64_t coded_end = (tk.flags & kTaskTailPad) ? 8ull * tk.S : vbits;
+  uint32_t x1 = 0, x2 = 0xffffffffu, acc = 0;
+  uint32_t opos = 0, ovf = 0;
+  for (uint64_t base = 0; base < coded_end; base += U) {
+    const uint32_t nvalid = base < vbits ? (uint32_t)min((uint64_t)U, vbits - base) : 0u;
+    if (nvalid) {
+      uint32_t sl[NJ][8], g[NJ][8], idx[NJ][8];
+      encode_issue<NJ>(M, src, L, base, U, nvalid, slot, tb, mask, Umax, lane, sl, g, idx);
+      clk.mark(2);
+      encode_predict<NJ>(M, T, nvalid, Umax, lane, sl, g, idx);
+      clk.mark(3);
+    }
+    // Arithmetic coder: a uniform (scalar) loop; (pr, y) of unit bit r is
+    // read from lane r & 63 of the register that holds bits 64j..64j+63.
+    uint32_t pv[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) pv[j] = (lane + 64u * j < Umax) ? M.rprm[lane + 64u * j] : 0u;
+    const uint32_t nc = (uint32_t)min((uint64_t)U, coded_end - base);
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const uint32_t lo = 64u * j;
+      const uint32_t hi = min(nc, lo + 64u);
+#pragma unroll 2
+      for (uint32_t r = lo; r < hi; ++r) {
+        const uint32_t prm = lane_read(pv[j], r - lo);
+        const uint32_t xmid = coder_mid(x1, x2, prm >> 1);
+        if (prm & 1u)
+          x2 = xmid;
+        else
+          x1 = xmid + 1u;
+        if (((x1 ^ x2) & 0xff000000u) == 0) {
+          // (rare) shift out the leading bytes x1 and x2 agree on
+          do {
+            acc |= (x2 >> 24) << (8 * (opos & 3u));
+            ++opos;
+            if ((opos & 3u) == 0) {
+              if (opos <= cap) {
+                if (lane == 0) *(GLB uint32_t *)(dst + opos - 4) = acc;
+              } else {
+                ovf = 1u;
+              }
+              acc = 0;
+            }
+            x1 <<= 8;
+            x2 = (x2 << 8) | 255u;
+          } while (((x1 ^ x2) & 0xff000000u) == 0);
+        }
+      }
+    }
+    clk.mark(4);
+    if (nvalid) update_unit<NJ>(M, T, slot, tb, Umax, nvalid, lane, clk);
+  }
+  acc |= (x1 >> 24) << (8 * (opos & 3u));
+  ++opos;
+  const uint64_t wstart = (uint64_t)((opos - 1) & ~3u);
+  if (wstart + 4 <= cap) {
+    if (lane == 0) *(GLB uint32_t *)(dst + wstart) = acc;
+  } else {
+    ovf = 1u;
+  }
+  if (lane == 0) a.result[t] = (uint64_t)opos | (ovf ? kResultOverflow : 0ull);
+  clk.flush(a.prof, lane);
+  wave_sync();
+}
+
+// ---------------------------------------------------------------------------
+// Decoder
+// ---------------------------------------------------------------------------
+
+// Window of the payload in registers: lane l holds bytes 4l..4l+3 after
+// winbase (0xff past the payload, as the reference decoder reads).
+CUCM_DEV uint32_t load_window(const GLB uint8_t *pay, uint64_t plen, uint64_t wb,
+                                                uint32_t lane) {
+  if (plen == 0) return 0xffffffffu;
+  // all four loads go out unconditionally (clamped into the payload), then
+  // the bytes past its end are replaced
+  uint32_t b[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint64_t at = wb + 4ull * lane + i;
+    b[i] = pay[at < plen ? at : plen - 1];
+  }
+  uint32_t w = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint64_t at = wb + 4ull * lane + i;
+    w |= (at < plen ? b[i] : 255u) << (8 * i);
+  }
+  return w;
+}
+
+// The 4 payload bytes at window offset off, as a big-endian word.
+CUCM_DEV uint32_t window_be32(uint32_t win, uint32_t off) {
+  const uint32_t w0 = lane_read(win, off >> 2);
+  const uint32_t w1 = lane_read(win, ((off >> 2) + 1u) & 63u);
+  const uint64_t v = (((uint64_t)w1 << 32) | w0) >> (8 * (off & 3u));
+  return __builtin_bswap32((uint32_t)v);
+}
+
+// Speculative paths of one decoder round of D bits: lane q + 64m evaluates
+// path q, which fixes the next plen(q) = floor(log2(q + 1)) bits of the
+// current byte to pbits(q) = q + 1 - 2^plen(q) (cucm_decompress_v2.cu:698-770).
+// The whole FNV chain of a path is hashed once in its lane; the 7 state
+// gathers are issued here and consumed by paths_finish, so the caller can do
+// other work while they are in flight.  Slots go to pslot[buf] for recording.
+template <int NM, uint32_t P>
+CUCM_DEV void paths_issue(const WaveMem &M, const GLB uint8_t *slot, uint32_t tb,
+                                            uint32_t mask, uint32_t j0, uint32_t partial,
+                                            uint64_t hist, uint32_t buf, uint32_t lane,
+                                            uint32_t g[NM][8]) {
+  LDS uint32_t *ps = M.pslot + buf * (P * 8u);
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const uint32_t q = lane + 64u * m;
+    const bool ok = q < P;
+    const uint32_t plen = 31u - clz32(q + 1u);
+    const uint32_t pbits = q + 1u - (1u << plen);
+    // (lanes without a path compute in-bounds garbage)
+    const uint32_t h0 = ((1u << (j0 + plen)) | (partial << plen) | pbits) & 255u;
+    uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
+    uint32_t sl[8];
+    sl[0] = h0;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      H = (H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u)) * kFnvPrime;
+      sl[k] = H & mask;
+    }
+    g[m][0] = M.st0[h0];
+    // unconditional loads (a predicated load would be waited for at the
+    // branch join); lanes without a path all read the slot's first byte
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+      g[m][k] = slot[ok ? ((uint64_t)(k - 1) << tb) + sl[k] : 0ull];
+    if (ok) {
+      LDS u32x4 *d = (LDS u32x4 *)(ps + q * 8u);
+      d[0] = u32x4{sl[0], sl[1], sl[2], sl[3]};
+      d[1] = u32x4{sl[4], sl[5], sl[6], sl[7]};
+    }
+  }
+}
+
+// Completes the paths of a round: statep, stretch, mixer context and mix
+// per path, all lane-local (cucm_decompress_v2.cu:774-809).  pc[m] receives
+// pr | cxt << 16; the unit-start states go to pg[buf] for recording.
+template <int NM, uint32_t P>
+CUCM_DEV void paths_finish(const WaveMem &M, const LDS DevTables *T, uint32_t c1,
+                                             uint32_t buf, uint32_t lane, const uint32_t g[NM][8],
+                                             uint32_t pc[NM]) {
+  LDS uint8_t *pgb = M.pg + buf * (P * 8u);
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const uint32_t q = lane + 64u * m;
+    uint32_t pv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pv[k] = M.stp[k * 256 + g[m][k]];
+    int32_t smp[8];
+    uint32_t active = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      smp[k] = T->stretch[pv[k] >> 20];
+      active += (smp[k] != 1);
+    }
+    const uint32_t cxt = active + c1 * 10u;
+    const LDS u32x4 *wr = (const LDS u32x4 *)(M.w + cxt * 8u);
+    const u32x4 wa = wr[0], wb = wr[1];
+    const int32_t w[8] = {(int32_t)wa.x, (int32_t)wa.y, (int32_t)wa.z, (int32_t)wa.w,
+                          (int32_t)wb.x, (int32_t)wb.y, (int32_t)wb.z, (int32_t)wb.w};
+    pc[m] = mix_weights(w, smp, T->squash) | (cxt << 16);
+    if (q < P) {
+      const uint32_t lo = g[m][0] | (g[m][1] << 8) | (g[m][2] << 16) | (g[m][3] << 24);
+      const uint32_t hi = g[m][4] | (g[m][5] << 8) | (g[m][6] << 16) | (g[m][7] << 24);
+      *(LDS uint64_t *)(pgb + q * 8u) = ((uint64_t)hi << 32) | lo;
+    }
+  }
+}
+
+// Path q's packed pr | cxt << 16 (uniform q).
+template <int NM>
+CUCM_DEV uint32_t path_value(const uint32_t pc[NM], uint32_t q) {
+  if (NM == 1) return lane_read(pc[0], q);
+  uint32_t v = lane_read(pc[0], q & 63u);
+#pragma unroll
+  for (int m = 1; m < NM; ++m) {
+    const uint32_t vm = lane_read(pc[m], q & 63u);
+    v = (q >> 6) == (uint32_t)m ? vm : v;
+  }
+  return v;
+}
+
+// One chunk.  Each round decides D bits: the paths of the round are mixed,
+// the bits decoded by a scalar coder loop, then the next round's gathers are
+// issued before the current round's bits are recorded for the unit update.
+template <int NJ, int D>
+CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
+                                            const WaveMem &M, const LDS DevTables *T,
+                                            GLB uint8_t *slot, uint32_t Umax, uint32_t lane) {
+  constexpr uint32_t P = (1u << D) - 1u;
+  constexpr int NM = (int)((P + 63u) / 64u);
+  const uint32_t U = tk.U, tb = tk.tb, mask = (1u << tb) - 1u;
+  const uint32_t TS = table_size(Umax);
+  const uint64_t L = tk.L;
+  const GLB uint8_t *pay = (const GLB uint8_t *)(uintptr_t)tk.src;
+  const uint64_t plen = tk.aux;
+  GLB uint8_t *dst = (GLB uint8_t *)(uintptr_t)tk.dst;
+  PhaseClock clk(a.prof != nullptr);
+  init_model(a, tk, M, slot, Umax, lane);
+  clk.mark(0);
+  uint64_t winbase = 0;
+  uint32_t win = load_window(pay, plen, 0, lane);
+  uint32_t x = __builtin_bswap32(lane_read(win, 0));
+  uint32_t x1 = 0, x2 = 0xffffffffu;
+  uint64_t ip = 4;
+  uint64_t hist = 0;     // the 8 bytes before the current one, latest lowest
+  uint32_t partial = 0;  // decided bits of the current byte
+  const uint64_t vbits = 8 * L;
+  for (uint64_t base = 0; base < vbits; base += U) {
+    // unit bits: a multiple of 8, so every round decides exactly D bits
+    const uint32_t nbits = (uint32_t)min((uint64_t)U, vbits - base);
+    uint32_t g[NM][8];
+    uint32_t buf = 0;
+    paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, buf, lane, g);
+    for (uint32_t rb = 0; rb < nbits; rb += D) {
+      // the coder consumes at most 4 bytes per bit
+      if (ip + 4 * D + 8 > winbase + 256) {
+        winbase = ip & ~3ull;
+        win = load_window(pay, plen, winbase, lane);
+      }
+      uint32_t pc[NM];
+      paths_finish<NM, P>(M, T, (uint32_t)(hist & 255u) >> 5, buf, lane, g, pc);
+      wave_sync();  // pg[buf] is read across lanes by the record step
+      clk.mark(9);
+      // decode the round's D bits (uniform scalar loop)
+      uint32_t pre = 0;
+      uint64_t chosen = 0, cxp = 0;
+      uint64_t prmp[2] = {0, 0};
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        const uint32_t q = (1u << i) - 1u + pre;
+        const uint32_t v = path_value<NM>(pc, q);
+        const uint32_t pr = v & 0xffffu, cx = v >> 16;
+        const uint32_t xmid = coder_mid(x1, x2, pr);
+        const uint32_t y = x <= xmid ? 1u : 0u;
+        if (y)
+          x2 = xmid;
+        else
+          x1 = xmid + 1u;
+        if (((x1 ^ x2) & 0xff000000u) == 0) {
+          // shift out every leading byte x1 and x2 agree on (at most 4)
+          const uint32_t nb = min((uint32_t)clz32(x1 ^ x2) >> 3, 4u);
+          const uint32_t sh = 8 * nb;
+          x1 = (uint32_t)((uint64_t)x1 << sh);
+          x2 = (uint32_t)(((uint64_t)x2 << sh) | ((1ull << sh) - 1ull));
+          const uint32_t nxt = window_be32(win, (uint32_t)(ip - winbase));
+          x = (uint32_t)((((uint64_t)x << 32) | nxt) >> (32 - sh));
+          ip += nb;
+        }
+        pre = (pre << 1) | y;
+        chosen |= (uint64_t)q << (8 * i);
+        prmp[i >> 2] |= (uint64_t)((pr << 1) | y) << (16 * (i & 3));
+        cxp |= (uint64_t)cx << (8 * i);
+      }
+      clk.mark(10);
+      const uint32_t j0 = rb & 7u, bi = rb >> 3;
+      partial = (partial << D) | pre;
+      if (j0 + D == 8) {
+        if (lane == 0) M.ub[8 + bi] = (uint8_t)partial;
+        hist = (hist << 8) | partial;
+        partial = 0;
+      }
+      // next round's gathers go out before this round is recorded
+      const uint32_t cur = buf;
+      buf ^= 1u;
+      if (rb + D < nbits) paths_issue<NM, P>(M, slot, tb, mask, (rb + D) & 7u, partial, hist, buf, lane, g);
+      // record the taken paths: slot and unit-start state of every order of
+      // every decided bit, and (pr, y), cxt per bit
+      {
+        const LDS uint8_t *pgb = M.pg + cur * (P * 8u);
+        const LDS uint32_t *psb = M.pslot + cur * (P * 8u);
+        const bool rec = lane < 8u * D;
+        const uint32_t i = lane >> 3, k = lane & 7u;
+        const uint32_t r = min(rb + i, Umax - 1);
+        const uint32_t q = (uint32_t)(chosen >> (8 * i)) & 255u;
+        const uint32_t gq = pgb[q * 8 + k];
+        const uint32_t sl = psb[q * 8 + k];
+        const bool ins = rec && k != 0;
+        uint32_t h = sl & (TS - 1);
+        LDS uint32_t *tkk = M.tkey + (k ? (k - 1) * TS : 0);
+        uint32_t old = lds_cas((ins ? &tkk[h] : &M.dummy[lane]), kEmpty, sl);
+        if (wave_ballot(ins && old != kEmpty && old != sl)) {
+          if (ins)
+            while (old != kEmpty && old != sl) {
+              h = (h + 1) & (TS - 1);
+              old = lds_cas(&tkk[h], kEmpty, sl);
+            }
+        }
+        if (rec) {
+          M.rg[k * Umax + r] = (uint8_t)gq;
+          M.ridx[k * Umax + r] = (uint16_t)(k == 0 ? sl : h);
+        }
+        if (lane < (uint32_t)D) {
+          const uint64_t pw = lane < 4 ? prmp[0] : prmp[1];
+          M.rprm[rb + lane] = (uint16_t)(pw >> (16 * (lane & 3u)));
+          M.rcxt[rb + lane] = (uint8_t)(cxp >> (8 * lane));
+        }
+      }
+      wave_sync();
+      clk.mark(11);
+    }
+    update_unit<NJ>(M, T, slot, tb, Umax, nbits, lane, clk);
+    const uint32_t nb = nbits >> 3;
+    for (uint32_t i = lane; i < nb; i += 64) dst[(base >> 3) + i] = M.ub[8 + i];
+    wave_sync();
+  }
+  if (lane == 0) a.result[t] = ip;
+  clk.flush(a.prof, lane);
+  wave_sync();
+}
+
+}  // namespace cucm
+
+#endif

@@ -1,0 +1,212 @@
+// wave_emulation.h -- CPU stand-ins for the cross-lane primitives of
+// cucm_wave.h (test infrastructure).
+//
+// The 64 lanes of a wave run as fibers on one host thread.  Each lane runs
+// until it reaches a cross-lane operation (readlane, readfirstlane, ballot,
+// wave_sync); when all 64 are there, the operation is resolved and every
+// lane continues.  Cross-lane LDS traffic is therefore ordered only at those
+// points, so device code that relies on lock step without a wave_sync shows
+// up as a mismatch here.  LDS and every global buffer are separate heap
+// blocks of their exact sizes, so AddressSanitizer reports any access out of
+// bounds (the GPU would silently drop an out-of-range LDS access).
+#ifndef CUCM_WAVE_EMULATION_H
+#define CUCM_WAVE_EMULATION_H
+#include <sanitizer/common_interface_defs.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <vector>
+
+#define LDS
+#define GLB
+#define CUCM_DEV inline
+#define CUCM_HD inline
+#define CUCM_HDI inline
+
+template <class T>
+inline T min(T a, T b) {
+  return a < b ? a : b;
+}
+template <class T>
+inline T max(T a, T b) {
+  return a > b ? a : b;
+}
+
+namespace emu {
+
+// Minimal x86-64 fiber switch: saves the callee-saved registers and the
+// stack pointer of the running fiber in *from, resumes the fiber whose
+// stack pointer is to.
+extern "C" void emu_fiber_swap(void **from, void *to);
+asm(R"(
+.text
+.globl emu_fiber_swap
+.type emu_fiber_swap,@function
+emu_fiber_swap:
+  pushq %rbp
+  pushq %rbx
+  pushq %r12
+  pushq %r13
+  pushq %r14
+  pushq %r15
+  movq %rsp, (%rdi)
+  movq %rsi, %rsp
+  popq %r15
+  popq %r14
+  popq %r13
+  popq %r12
+  popq %rbx
+  popq %rbp
+  ret
+.size emu_fiber_swap, .-emu_fiber_swap
+)");
+
+constexpr int kLanes = 64;
+constexpr size_t kStack = 1 << 20;
+enum Kind { kNone, kRead, kFirst, kBallot, kSync };
+
+struct Wave {
+  void *sched_sp = nullptr;
+  void *sp[kLanes];
+  std::vector<char> stack[kLanes];
+  bool done[kLanes];
+  Kind kind[kLanes];
+  uint64_t val[kLanes];
+  int line[kLanes];
+  uint64_t snap[kLanes];  // values of the resolved operation
+  uint64_t ballot = 0;
+  int cur = 0;
+  const void *sched_bottom = nullptr;
+  size_t sched_size = 0;
+  std::function<void(uint32_t)> body;
+};
+
+inline Wave *&current() {
+  static Wave *w = nullptr;
+  return w;
+}
+
+// Switches from a lane back to the scheduler (AddressSanitizer is told
+// which stack becomes current).
+inline void to_sched(Wave &w, int l) {
+  void *fake = nullptr;
+  __sanitizer_start_switch_fiber(&fake, w.sched_bottom, w.sched_size);
+  emu_fiber_swap(&w.sp[l], w.sched_sp);
+  __sanitizer_finish_switch_fiber(fake, nullptr, nullptr);
+}
+
+inline void to_lane(Wave &w, int l) {
+  void *fake = nullptr;
+  w.cur = l;
+  __sanitizer_start_switch_fiber(&fake, w.stack[l].data(), w.stack[l].size());
+  emu_fiber_swap(&w.sched_sp, w.sp[l]);
+  __sanitizer_finish_switch_fiber(fake, &w.sched_bottom, &w.sched_size);
+}
+
+extern "C" inline void emu_lane_entry() {
+  __sanitizer_finish_switch_fiber(nullptr, nullptr, nullptr);
+  Wave &w = *current();
+  const int l = w.cur;
+  w.body((uint32_t)l);
+  w.done[l] = true;
+  w.kind[l] = kNone;
+  to_sched(w, l);
+  abort();  // a finished lane is never resumed
+}
+
+// Lane side of a cross-lane operation: publish, yield, read the result.
+inline uint64_t meet(Kind k, uint64_t v, uint32_t arg, int line) {
+  Wave &w = *current();
+  const int l = w.cur;
+  w.kind[l] = k;
+  w.val[l] = v;
+  w.line[l] = line;
+  to_sched(w, l);
+  switch (k) {
+    case kRead: return w.snap[arg & 63u];
+    case kFirst: return w.snap[0];
+    case kBallot: return w.ballot;
+    default: return 0;
+  }
+}
+
+// Runs body(lane) for the 64 lanes of one wave to completion.
+inline void run_wave(const std::function<void(uint32_t)> &body) {
+  Wave *w = new Wave();
+  current() = w;
+  w->body = body;
+  for (int l = 0; l < kLanes; ++l) {
+    w->stack[l].assign(kStack, 0);
+    w->done[l] = false;
+    // initial frame: six zero registers, then the entry as return address
+    uintptr_t top = ((uintptr_t)(w->stack[l].data() + kStack) & ~(uintptr_t)15) - 64;
+    void **f = (void **)top;
+    for (int i = 0; i < 6; ++i) f[i] = nullptr;
+    f[6] = (void *)&emu_lane_entry;
+    f[7] = nullptr;
+    w->sp[l] = (void *)top;
+  }
+  for (;;) {
+    for (int l = 0; l < kLanes; ++l)
+      if (!w->done[l]) to_lane(*w, l);
+    int ndone = 0;
+    for (int l = 0; l < kLanes; ++l) ndone += w->done[l];
+    if (ndone == kLanes) break;
+    if (ndone) {
+      int at = 0;
+      while (w->done[at]) ++at;
+      fprintf(stderr, "wave emulation: %d lanes finished while others wait at line %d\n", ndone,
+              w->line[at]);
+      abort();
+    }
+    for (int l = 1; l < kLanes; ++l)
+      if (w->kind[l] != w->kind[0] || w->line[l] != w->line[0]) {
+        fprintf(stderr,
+                "wave emulation: divergent cross-lane operation: lane 0 at line %d, lane %d at "
+                "line %d\n",
+                w->line[0], l, w->line[l]);
+        abort();
+      }
+    w->ballot = 0;
+    for (int l = 0; l < kLanes; ++l) {
+      w->snap[l] = w->val[l];
+      if (w->val[l]) w->ballot |= 1ull << l;
+    }
+  }
+  current() = nullptr;
+  delete w;
+}
+
+}  // namespace emu
+
+namespace cucm {
+#define CUCM_EMU_MEET(k, v, a) emu::meet(k, v, a, __LINE__)
+inline uint32_t lane_read_at(uint32_t v, uint32_t lane, int line) {
+  return (uint32_t)emu::meet(emu::kRead, v, lane, line);
+}
+#define lane_read(v, l) lane_read_at((v), (l), __LINE__)
+inline uint32_t lane_first_at(uint32_t v, int line) {
+  return (uint32_t)emu::meet(emu::kFirst, v, 0, line);
+}
+#define lane_first(v) lane_first_at((v), __LINE__)
+inline uint64_t wave_ballot_at(bool p, int line) { return emu::meet(emu::kBallot, p ? 1 : 0, 0, line); }
+#define wave_ballot(p) wave_ballot_at((p), __LINE__)
+inline void wave_sync_at(int line) { emu::meet(emu::kSync, 0, 0, line); }
+#define wave_sync() wave_sync_at(__LINE__)
+#define vm_drain() wave_sync_at(__LINE__)
+inline uint32_t clz32(uint32_t x) { return x ? (uint32_t)__builtin_clz(x) : 32u; }
+inline uint64_t shader_clock() { return 0; }
+inline void wait_vm() {}
+inline void wait_lgkm() {}
+inline uint32_t lds_cas(uint32_t *p, uint32_t cmp, uint32_t v) {
+  const uint32_t o = *p;
+  if (o == cmp) *p = v;
+  return o;
+}
+inline void lds_add(int32_t *p, int32_t v) { *p = (int32_t)((uint32_t)*p + (uint32_t)v); }
+inline void glb_add64(uint64_t *p, uint64_t v) { *p += v; }
+}  // namespace cucm
+
+#endif
