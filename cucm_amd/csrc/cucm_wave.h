@@ -49,6 +49,8 @@ CUCM_DEV uint64_t wave_ballot(bool p) { return __ballot(p); }
 CUCM_DEV uint32_t clz32(uint32_t x) { return (uint32_t)__clz((int)x); }  // 32 for 0
 CUCM_DEV uint64_t shader_clock() { return clock64(); }
 CUCM_DEV void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0f70); }    // vmcnt(0)
+// Keeps the instruction scheduler from moving code across this point.
+CUCM_DEV void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 CUCM_DEV void wait_lgkm() { __builtin_amdgcn_s_waitcnt(0xc07f); }  // lgkmcnt(0)
 // Orders the wave's LDS traffic across lanes (and stops the compiler from
 // moving LDS accesses across it); lanes of a wave run in lock step.
@@ -261,7 +263,7 @@ CUCM_DEV void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M, G
   wave_sync();
 }
 
-// Applies the deferred updates of one unit (cucm_compress_v2.cu:794-839,
+// First part of the deferred update of one unit (cucm_compress_v2.cu:794-839,
 // cucm_decompress_v2.cu:870-896), for unit bits r < nvalid (a multiple of 8):
 //   1. mixer deltas from the recorded (pr, y, cxt) and unit-start predictions
 //      (LDS integer atomics: the sum is order independent);
@@ -270,17 +272,17 @@ CUCM_DEV void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M, G
 //      share an order-0 slot (distinct partial bytes), so a byte's 8 bits x
 //      8 orders go in one step over all 64 lanes; a byte where two bits of an
 //      order hash to one slot (a hash collision) is stepped bit by bit;
-//   3. per order, the statep recurrence in bit order (8 lanes), with the next
-//      bit's statep read issued before the current write and forwarded when
-//      both hit the same state;
-//   4. the final state of every touched slot back to HBM.
+//   3. the final state of every touched slot back to HBM, and the tables
+//      cleared for the next unit.
+// ym receives the unit's bits y as uniform masks for the statep walk (the
+// second part, the statep walk), which may run while other work proceeds.
 template <int NJ>
-CUCM_DEV void update_unit(const WaveMem &M, const LDS DevTables *T, GLB uint8_t *slot, uint32_t tb,
-                            uint32_t Umax, uint32_t nvalid, uint32_t lane, PhaseClock &clk) {
+CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_t *slot,
+                            uint32_t tb, uint32_t Umax, uint32_t nvalid, uint32_t lane,
+                            PhaseClock &clk, uint64_t ym[NJ]) {
   const uint32_t TS = table_size(Umax);
   const uint32_t nbytes = nvalid >> 3;
   // y of every unit bit as a uniform bit mask
-  uint64_t ym[NJ];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const uint32_t r = lane + 64u * j;
@@ -362,67 +364,7 @@ CUCM_DEV void update_unit(const WaveMem &M, const LDS DevTables *T, GLB uint8_t 
     }
   }
   clk.mark(6);
-  // 3. statep recurrences: lane k < 8 walks order k's updates in bit order.
-  // Software-pipelined so that the dependent chain per bit is the update
-  // arithmetic alone: the statep entry of bit i+2 is read before bit i's
-  // write (the two latest writes are forwarded instead), and the adaptation
-  // rate dt[n] of every candidate is looked up a bit early -- the count n in
-  // an entry's low 10 bits advances to min(n + 1, 1023) on every update,
-  // whatever the bit, so it never waits for the arithmetic.
-  if (lane < 8) {
-    LDS uint32_t *sp = M.stp + lane * 256;
-    const LDS uint16_t *dtab = T->dt;
-    const LDS u32x4 *S4 = (const LDS u32x4 *)(M.rg + lane * Umax);
-    const uint32_t nblk = (nvalid + 15) >> 4;
-    u32x4 cur = S4[0];
-    u32x4 nxt = nblk > 1 ? S4[1] : cur;
-    uint32_t s0 = cur.x & 255u, s1 = (cur.x >> 8) & 255u, sprev = 0x100u;
-    uint32_t p = sp[s0];
-    uint32_t d = dtab[p & 1023u];
-    uint32_t raw1 = sp[s1];  // read before any write of this unit
-    uint32_t pnprev = 0, dfprev = 0;
-    for (uint32_t blk = 0; blk < nblk; ++blk) {
-      const uint32_t seg = blk >> 2;
-      uint64_t yw = ym[0];
-#pragma unroll
-      for (int q = 1; q < NJ; ++q)
-        if (seg == (uint32_t)q) yw = ym[q];
-      yw >>= 16 * (blk & 3u);
-#pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        if (blk * 16 + t < nvalid) {
-          const int t2 = t + 2;
-          const uint32_t w2 = t2 < 16 ? cur[t2 >> 2] : nxt[(t2 - 16) >> 2];
-          const uint32_t s2 = (w2 >> (8 * (t2 & 3))) & 255u;
-          const uint32_t raw2 = sp[s2];  // misses the writes of bits i, i+1
-          const uint32_t n = p & 1023u;
-          const uint32_t nn = min(n + 1u, 1023u);
-          const uint32_t dfwd = dtab[nn];
-          const uint32_t draw = dtab[raw1 & 1023u];
-          const uint32_t y22 = (uint32_t)((yw >> t) & 1u) << 22;
-          const int32_t e = (int32_t)(y22 - (p >> 10));
-          const uint32_t pn = p + (((uint32_t)(e >> 3) * d) & 0xfffffc00u) + (nn - n);
-          sp[s0] = pn;
-          const bool same1 = s1 == s0, same2 = s1 == sprev;
-          const uint32_t pnext = same1 ? pn : (same2 ? pnprev : raw1);
-          const uint32_t dnext = same1 ? dfwd : (same2 ? dfprev : draw);
-          sprev = s0;
-          s0 = s1;
-          s1 = s2;
-          pnprev = pn;
-          dfprev = dfwd;
-          raw1 = raw2;
-          p = pnext;
-          d = dnext;
-        }
-      }
-      cur = nxt;
-      nxt = (blk + 2 < nblk) ? S4[blk + 2] : cur;
-    }
-  }
-  wave_sync();
-  clk.mark(7);
-  // 4. write back the touched slots (every occurrence writes the same final
+  // 3. write back the touched slots (every occurrence writes the same final
   // state), then clear the tables
 #pragma unroll 1
   for (uint32_t k = 1; k < 8; ++k) {
@@ -443,6 +385,122 @@ CUCM_DEV void update_unit(const WaveMem &M, const LDS DevTables *T, GLB uint8_t 
   vm_drain();
   wave_sync();
   clk.mark(8);
+}
+
+// Second part of the deferred update: the per-order statep recurrences
+// (cucm_compress_v2.cu:588-601 applied in bit order).  Lane k < 8 walks order
+// k's updates; lanes 8..63 run the same instructions on order 7's row and
+// write a dummy word, so a walk step has no divergent branch and can share a
+// loop with other work.  Software-pipelined so that the dependent chain per
+// bit is the update arithmetic alone: the statep entry of bit i+2 is read
+// before bit i's write (the two latest writes are forwarded instead), and
+// the adaptation rate dt[n] of every candidate is looked up one bit before it
+// is needed -- the count n in an entry's low 10 bits advances to
+// min(n + 1, 1023) on every update, whatever the bit.  Steps go in blocks of
+// 8 bits (unit bit counts are multiples of 8).
+// The walk's state lives in plain locals of the function running it (a
+// struct or a member function would be optimised as memory before inlining,
+// and a select between its fields would then keep it out of registers).
+#define CUCM_WALK_STATE                                                            \
+  LDS uint32_t *w_sp = nullptr;                                                    \
+  LDS uint32_t *w_dummy = nullptr;                                                 \
+  const LDS uint16_t *w_dtab = nullptr;                                            \
+  const LDS uint8_t *w_sv = nullptr;                                               \
+  bool w_writer = false, w_f1 = true, w_f2 = false;                                \
+  uint32_t w_nblk = 0, w_blk = 0;                                                  \
+  uint64_t w_cur = 0, w_nxt = 0;                                                   \
+  uint32_t w_s0 = 0, w_s1 = 0, w_sprev = 0x100u, w_p = 0, w_raw1 = 0, w_pnprev = 0, \
+           w_dfprev = 0, w_dA = 0, w_dB = 0, w_dC = 0
+
+#define CUCM_WALK_INIT(M, T, Umax, nvalid, lane)                             \
+  do {                                                                       \
+    const uint32_t wk_ = min((lane), 7u);                                    \
+    w_sp = (M).stp + wk_ * 256;                                              \
+    w_dummy = (M).dummy + (lane);                                            \
+    w_dtab = (T)->dt;                                                        \
+    w_sv = (M).rg + wk_ * (Umax);                                            \
+    w_writer = (lane) < 8;                                                   \
+    w_nblk = (nvalid) >> 3;                                                  \
+    w_blk = 0;                                                               \
+    w_cur = *(const LDS uint64_t *)w_sv;                                     \
+    w_nxt = w_nblk > 1 ? *(const LDS uint64_t *)(w_sv + 8) : w_cur;          \
+    w_s0 = (uint32_t)w_cur & 255u;                                           \
+    w_s1 = (uint32_t)(w_cur >> 8) & 255u;                                    \
+    w_sprev = 0x100u;                                                        \
+    w_p = w_sp[w_s0];                                                        \
+    w_raw1 = w_sp[w_s1]; /* read before any write of this unit */           \
+    w_dA = w_dtab[w_p & 1023u];                                              \
+    w_dB = w_dC = 0;                                                         \
+    w_f1 = true;                                                             \
+    w_f2 = false;                                                            \
+    w_pnprev = w_dfprev = 0;                                                 \
+  } while (0)
+
+// Bit t (a constant, 0..7) of the current block, with its y.
+#define CUCM_WALK_STEP(t, y)                                                               \
+  do {                                                                                     \
+    const uint32_t wd_ = w_f1 ? w_dA : (w_f2 ? w_dB : w_dC);                               \
+    const int wt2_ = (t) + 2;                                                              \
+    const uint64_t ww_ = wt2_ < 8 ? w_cur : w_nxt;                                         \
+    const uint32_t ws2_ = (uint32_t)(ww_ >> (8 * (wt2_ & 7))) & 255u;                      \
+    const uint32_t wraw2_ = w_sp[ws2_]; /* misses the writes of bits i, i+1 */            \
+    const uint32_t wn_ = w_p & 1023u;                                                      \
+    const uint32_t wnn_ = min(wn_ + 1u, 1023u);                                            \
+    const uint32_t wdfwd_ = w_dtab[wnn_];                                                  \
+    const uint32_t wdraw_ = w_dtab[w_raw1 & 1023u];                                        \
+    const int32_t we_ = (int32_t)(((uint32_t)(y) << 22) - (w_p >> 10));                   \
+    const uint32_t wpn_ = w_p + (((uint32_t)(we_ >> 3) * wd_) & 0xfffffc00u) + (wnn_ - wn_); \
+    *(w_writer ? &w_sp[w_s0] : w_dummy) = wpn_;                                            \
+    const bool wsame1_ = w_s1 == w_s0, wsame2_ = w_s1 == w_sprev;                          \
+    w_p = wsame1_ ? wpn_ : (wsame2_ ? w_pnprev : w_raw1);                                  \
+    w_dA = wdfwd_;                                                                         \
+    w_dB = w_dfprev;                                                                       \
+    w_dC = wdraw_;                                                                         \
+    w_f1 = wsame1_;                                                                        \
+    w_f2 = wsame2_;                                                                        \
+    w_sprev = w_s0;                                                                        \
+    w_s0 = w_s1;                                                                           \
+    w_s1 = ws2_;                                                                           \
+    w_pnprev = wpn_;                                                                       \
+    w_dfprev = wdfwd_;                                                                     \
+    w_raw1 = wraw2_;                                                                       \
+    sched_fence(); /* each step's loads stay ahead of the next step's uses */              \
+  } while (0)
+
+#define CUCM_WALK_NEXT_BLOCK()                                                            \
+  do {                                                                                    \
+    ++w_blk;                                                                              \
+    w_cur = w_nxt;                                                                        \
+    w_nxt = (w_blk + 1 < w_nblk) ? *(const LDS uint64_t *)(w_sv + 8 * (w_blk + 1)) : w_cur; \
+  } while (0)
+
+// y of unit bit r from the masks update_states returns.
+template <int NJ>
+CUCM_DEV uint32_t unit_y(const uint64_t ym[NJ], uint32_t r) {
+  uint64_t w = ym[0];
+#pragma unroll
+  for (int q = 1; q < NJ; ++q)
+    if ((r >> 6) == (uint32_t)q) w = ym[q];
+  return (uint32_t)(w >> (r & 63u)) & 1u;
+}
+
+// The whole statep walk of a unit on its own.
+template <int NJ>
+CUCM_DEV void statep_walk(const WaveMem &M, const LDS DevTables *T, uint32_t Umax, uint32_t nvalid,
+                          uint32_t lane, const uint64_t ym[NJ]) {
+  CUCM_WALK_STATE;
+  CUCM_WALK_INIT(M, T, Umax, nvalid, lane);
+  for (uint32_t blk = 0; blk < nvalid / 8; ++blk) {
+    uint64_t w = ym[0];
+#pragma unroll
+    for (int q = 1; q < NJ; ++q)
+      if ((blk >> 3) == (uint32_t)q) w = ym[q];
+    const uint32_t yb = (uint32_t)(w >> (8 * (blk & 7u))) & 255u;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) CUCM_WALK_STEP(t, (yb >> t) & 1u);
+    CUCM_WALK_NEXT_BLOCK();
+  }
+  wave_sync();
 }
 
 // ---------------------------------------------------------------------------
@@ -579,8 +637,38 @@ CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
   wave_sync();
 }
 
-// One chunk.  Per unit: stage, hash and gather, predictions, the arithmetic
-// coder, then the deferred update.
+// One step of the carry-less binary arithmetic coder (cucm_compress_v2.cu:
+// 841-868) for the packed prm = pr << 1 | y, in the uniform registers x1, x2;
+// the rare byte output goes through acc (4 bytes per store) to dst.
+#define CUCM_CODE_BIT(prm_expr)                                           \
+  do {                                                                    \
+    const uint32_t prm_ = (prm_expr);                                     \
+    const uint32_t xmid_ = coder_mid(x1, x2, prm_ >> 1);                  \
+    if (prm_ & 1u)                                                        \
+      x2 = xmid_;                                                         \
+    else                                                                  \
+      x1 = xmid_ + 1u;                                                    \
+    if (((x1 ^ x2) & 0xff000000u) == 0) {                                 \
+      /* (rare) shift out the leading bytes x1 and x2 agree on */         \
+      do {                                                                \
+        acc |= (x2 >> 24) << (8 * (opos & 3u));                           \
+        ++opos;                                                           \
+        if ((opos & 3u) == 0) {                                           \
+          if (opos <= cap) {                                              \
+            if (lane == 0) *(GLB uint32_t *)(dst + opos - 4) = acc;       \
+          } else {                                                        \
+            ovf = 1u;                                                     \
+          }                                                               \
+          acc = 0;                                                        \
+        }                                                                 \
+        x1 <<= 8;                                                         \
+        x2 = (x2 << 8) | 255u;                                            \
+      } while (((x1 ^ x2) & 0xff000000u) == 0);                           \
+    }                                                                     \
+  } while (0)
+
+// One chunk.  Per unit: stage, hash and gather, predictions, the first part
+// of the deferred update, then the arithmetic coder and the statep walk.
 template <int NJ>
 CUCM_DEV void encode_task(const CodecArgs &a, const Task &tk, uint32_t t,
                                             const WaveMem &M, const LDS DevTables *T,
@@ -606,45 +694,39 @@ CUCM_DEV void encode_task(const CodecArgs &a, const Task &tk, uint32_t t,
       encode_predict<NJ>(M, T, nvalid, Umax, lane, sl, g, idx);
       clk.mark(3);
     }
-    // Arithmetic coder: a uniform (scalar) loop; (pr, y) of unit bit r is
-    // read from lane r & 63 of the register that holds bits 64j..64j+63.
+    // The deferred update's first part, then the arithmetic coder and the
+    // statep walk in one loop: the coder (a uniform scalar chain; (pr, y) of
+    // unit bit r is read from lane r & 63 of the register holding bits
+    // 64j..64j+63) and the walk (a vector chain) fill each other's stalls.
+    uint64_t ym[NJ] = {};
+    CUCM_WALK_STATE;
+    if (nvalid) {
+      update_states<NJ>(M, T, slot, tb, Umax, nvalid, lane, clk, ym);
+      CUCM_WALK_INIT(M, T, Umax, nvalid, lane);
+    }
     uint32_t pv[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) pv[j] = (lane + 64u * j < Umax) ? M.rprm[lane + 64u * j] : 0u;
     const uint32_t nc = (uint32_t)min((uint64_t)U, coded_end - base);
+    const uint32_t nwalk = nvalid >> 3;
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       const uint32_t lo = 64u * j;
       const uint32_t hi = min(nc, lo + 64u);
-#pragma unroll 2
-      for (uint32_t r = lo; r < hi; ++r) {
-        const uint32_t prm = lane_read(pv[j], r - lo);
-        const uint32_t xmid = coder_mid(x1, x2, prm >> 1);
-        if (prm & 1u)
-          x2 = xmid;
-        else
-          x1 = xmid + 1u;
-        if (((x1 ^ x2) & 0xff000000u) == 0) {
-          // (rare) shift out the leading bytes x1 and x2 agree on
-          do {
-            acc |= (x2 >> 24) << (8 * (opos & 3u));
-            ++opos;
-            if ((opos & 3u) == 0) {
-              if (opos <= cap) {
-                if (lane == 0) *(GLB uint32_t *)(dst + opos - 4) = acc;
-              } else {
-                ovf = 1u;
-              }
-              acc = 0;
-            }
-            x1 <<= 8;
-            x2 = (x2 << 8) | 255u;
-          } while (((x1 ^ x2) & 0xff000000u) == 0);
+      const uint32_t hw = min(hi, max(nwalk * 8u, lo));  // bits with a walk step
+      for (uint32_t r0 = lo; r0 < hw; r0 += 8) {
+        const uint32_t yb = (uint32_t)(ym[j] >> (r0 - lo)) & 255u;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          CUCM_WALK_STEP(t, (yb >> t) & 1u);
+          CUCM_CODE_BIT(lane_read(pv[j], r0 - lo + t));
         }
+        CUCM_WALK_NEXT_BLOCK();
       }
+      for (uint32_t r = hw; r < hi; ++r) CUCM_CODE_BIT(lane_read(pv[j], r - lo));
     }
+    if (nvalid) wave_sync();  // statep written by the walk is read across lanes next
     clk.mark(4);
-    if (nvalid) update_unit<NJ>(M, T, slot, tb, Umax, nvalid, lane, clk);
   }
   acc |= (x1 >> 24) << (8 * (opos & 3u));
   ++opos;
@@ -809,12 +891,14 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
   uint64_t hist = 0;     // the 8 bytes before the current one, latest lowest
   uint32_t partial = 0;  // decided bits of the current byte
   const uint64_t vbits = 8 * L;
+  // the first round's gathers of every unit are issued before the previous
+  // unit's statep walk (they read only the states the write-back stored)
+  uint32_t g[NM][8];
+  if (vbits) paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, 0, lane, g);
   for (uint64_t base = 0; base < vbits; base += U) {
     // unit bits: a multiple of 8, so every round decides exactly D bits
     const uint32_t nbits = (uint32_t)min((uint64_t)U, vbits - base);
-    uint32_t g[NM][8];
     uint32_t buf = 0;
-    paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, buf, lane, g);
     for (uint32_t rb = 0; rb < nbits; rb += D) {
       // the coder consumes at most 4 bytes per bit
       if (ip + 4 * D + 8 > winbase + 256) {
@@ -902,10 +986,13 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
       wave_sync();
       clk.mark(11);
     }
-    update_unit<NJ>(M, T, slot, tb, Umax, nbits, lane, clk);
+    uint64_t ym[NJ];
+    update_states<NJ>(M, T, slot, tb, Umax, nbits, lane, clk, ym);
     const uint32_t nb = nbits >> 3;
     for (uint32_t i = lane; i < nb; i += 64) dst[(base >> 3) + i] = M.ub[8 + i];
-    wave_sync();
+    if (base + U < vbits) paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, 0, lane, g);
+    statep_walk<NJ>(M, T, Umax, nbits, lane, ym);
+    clk.mark(7);
   }
   if (lane == 0) a.result[t] = ip;
   clk.flush(a.prof, lane);

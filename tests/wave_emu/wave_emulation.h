@@ -199,6 +199,7 @@ inline void wave_sync_at(int line) { emu::meet(emu::kSync, 0, 0, line); }
 inline uint32_t clz32(uint32_t x) { return x ? (uint32_t)__builtin_clz(x) : 32u; }
 inline uint64_t shader_clock() { return 0; }
 inline void wait_vm() {}
+inline void sched_fence() {}
 inline void wait_lgkm() {}
 inline uint32_t lds_cas(uint32_t *p, uint32_t cmp, uint32_t v) {
   const uint32_t o = *p;
