@@ -14,8 +14,11 @@ after the timed region: the round trip must be lossless, and the payloads of
 a sample of chunks must equal the oracle's (the CPU restatement of the
 reference), which is also timed as the CPU baseline.
 
-Other workloads: --workload batch (10,000 lognormal files, batched vs
-plain launch), --workload serve (Poisson request mix, p50/p99).
+Other workloads: --workload structured (1 GiB structured binary, 256 KiB
+chunks, 2^22 slots), --workload small (1 MiB order-2 Markov, 16 KiB chunks,
+2^18 slots), --workload batch (10,000 lognormal files, batched vs plain
+launch), --workload serve (Poisson request mix, p50/p99), --workload gather
+(random-touch ceiling), --workload copy (counter calibration).
 """
 import argparse
 import json
@@ -41,15 +44,32 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# Single-stream workloads: (generator, default MiB, chunk size, table bits, description).
+STREAMS = {
+    "stream": ("mixed", 256, 1 * MiB, 22,
+               "mixed stream %d MiB (60%% order-3 text, 25%% LZ copies, 15%% random 4 KiB "
+               "blocks; seed 2), 1 MiB chunks, 2^22 slots/order, U=128, P=0"),
+    "structured": ("structured", 1024, 256 * 1024, 22,
+                   "structured binary %d MiB (int32 walks, float32 fields, 16-byte records; "
+                   "seed 4), 256 KiB chunks, 2^22 slots/order, U=128, P=0"),
+    "small": ("markov2", 1, 16 * 1024, 18,
+              "order-2 Markov %d MiB (64 symbols, Zipf 1.1; seed 1), 16 KiB chunks, "
+              "2^18 slots/order, U=128, P=0"),
+}
+
+
 def stream_workload(args):
     import cucm_amd
     from cucm_amd import datagen
 
-    n = args.mib * MiB
-    S, U, tb, batch = 1 * MiB, 128, 22, 100
+    gen, mib_default, S, tb, desc = STREAMS[args.workload]
+    mib = args.mib or mib_default
+    n = mib * MiB
+    U, batch = 128, 100
     t0 = time.time()
-    data = datagen.mixed(n, seed=2)
-    log("generated %d MiB mixed stream in %.1fs" % (args.mib, time.time() - t0))
+    seed = {"mixed": 2, "structured": 4, "markov2": 1}[gen]
+    data = getattr(datagen, gen)(n, seed)
+    log("generated %d MiB %s input in %.1fs" % (mib, gen, time.time() - t0))
     ctx = cucm_amd.Context()
     params = cucm_amd.make_params(S, U, batch, 0, tb)
     C = (n + S - 1) // S
@@ -127,9 +147,8 @@ def stream_workload(args):
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32 fixed-point",
         "data": "synthetic (seeded generator, resident in HBM)",
-        "config": {"workload": "mixed stream %d MiB, 1 MiB chunks, 2^22 slots/order, U=128, P=0"
-                   % args.mib, "chunks": C, "chunk_size": S, "unit_bits": U, "table_bits": tb,
-                   "batch_chunks": batch, "spec_bits": args.spec},
+        "config": {"workload": desc % mib, "chunks": C, "chunk_size": S, "unit_bits": U,
+                   "table_bits": tb, "batch_chunks": batch, "spec_bits": args.spec},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     line.update(res)
@@ -183,7 +202,7 @@ def cpu_baseline(data, container, S, U, tb, args):
     from oracle import oracle as O
 
     threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-    k = args.cpu_chunks or threads
+    k = min(args.cpu_chunks or threads, (len(data) + S - 1) // S)
     sample = data[: k * S].tobytes()
     model = O.Model(tb)
     t0 = time.perf_counter()
@@ -204,8 +223,9 @@ def cpu_baseline(data, container, S, U, tb, args):
         prev = e
     cpu = {"value": len(sample) / 1e6 / (t1 - t0), "unit": "MB/s (compress)",
            "decompress_MBps": len(sample) / 1e6 / (t2 - t1), "cores": threads, "kind": "port",
-           "sample": "%d chunks x %d MiB of the same stream (first %d MiB), oracle encode_batch/"
-                     "decode_batch, OpenMP one chunk per core" % (k, S // MiB, k * S // MiB),
+           "sample": "%d chunks x %d KiB of the same input (its first %d KiB), oracle "
+                     "encode_batch/decode_batch, OpenMP one chunk per core" % (k, S // 1024,
+                                                                              len(sample) // 1024),
            "lossless": ok}
     return cpu, same
 
@@ -216,8 +236,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="stream",
-                    choices=["stream", "batch", "serve", "gather", "copy"])
-    ap.add_argument("--mib", type=int, default=256)
+                    choices=["stream", "structured", "small", "batch", "serve", "gather", "copy"])
+    ap.add_argument("--mib", type=int, default=0, help="input size (0 = the workload's own)")
     ap.add_argument("--spec", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -233,7 +253,7 @@ def main():
         print("bench.py measures one GPU only: the codec shards by chunk with no exchange on the "
               "hot loop, and this build runs on a single MI355X (use --gpus 1).", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "stream":
+    if args.workload in STREAMS:
         line = stream_workload(args)
     elif args.workload == "batch":
         from cucm_amd.workloads import batch_workload
