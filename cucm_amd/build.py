@@ -2,6 +2,7 @@
 
   libcucm.so          HIP kernels (gfx950) + host engine + C ABI (include/cucm.h)
   libcucm_datagen.so  seeded synthetic input generators (host, OpenMP)
+  bin/cucm_compress, bin/cucm_decompress   drop-in command-line tools
 
 Run:  python -m cucm_amd.build   (or __graft_entry__.build()).
 """
@@ -18,9 +19,11 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CUCM_ARCH", "gfx950")
 
 LIB = os.path.join(HERE, "libcucm.so")
+BIN = os.path.join(HERE, "bin")
+CLIS = ["cucm_compress", "cucm_decompress"]
 DATAGEN = os.path.join(HERE, "libcucm_datagen.so")
 
-HIP_SOURCES = ["cucm_kernels.hip", "cucm_engine.cpp", "cucm_host_model.cpp", "cucm_serve.cpp",
+HIP_SOURCES = ["cucm_kernels.hip", "cucm_engine.cpp", "cucm_host_model.cpp", "cucm_serve.cpp", "cucm_stream.cpp",
                "gather_probe.hip"]
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
@@ -59,6 +62,12 @@ def build(verbose=False):
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", LIB + ".tmp"] + [o for _, o in jobs])
     os.replace(LIB + ".tmp", LIB)
     os.replace(DATAGEN + ".tmp", DATAGEN)
+    # drop-in command-line tools: host C++ over the C ABI
+    os.makedirs(BIN, exist_ok=True)
+    for tool in CLIS:
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+              os.path.join(HERE, "cli", tool + ".cpp"), "-o", os.path.join(BIN, tool),
+              "-L" + HERE, "-lcucm", "-Wl,-rpath,$ORIGIN/.."])
     return LIB
 
 

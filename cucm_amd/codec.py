@@ -260,6 +260,72 @@ class Context:
                   "decompress_device")
         return list(olen), list(st)
 
+    # ---- streaming container I/O ----------------------------------------
+    def compress_stream(self, data, params, segment_batches=1):
+        """Compresses through the streaming C ABI (header, then segments of
+        `segment_batches` batch records each); returns the container, which
+        equals compress()'s."""
+        addr, N, keep = _buf(data)
+        P = params.prelearn_size
+        head = np.empty(21 + P, dtype=np.uint8)
+        hl = ctypes.c_uint64()
+        s = ctypes.c_void_p()
+        check(lib().cucm_stream_compress_begin(self.handle, N, ctypes.byref(params), addr,
+                                               head.ctypes.data, head.size, ctypes.byref(hl),
+                                               ctypes.byref(s)), "stream_compress_begin")
+        parts = [head[: hl.value].tobytes()]
+        q = make_params(params.chunk_size, params.unit_bits, params.batch_chunks, 0,
+                        params.table_bits, tail_pad=not (params.flags & FLAG_NO_TAIL_PAD))
+        seg = params.batch_chunks * params.chunk_size * max(1, segment_batches)
+        try:
+            pos = P
+            while pos < N:
+                n = min(seg, N - pos)
+                out = np.empty(output_cap(n, q), dtype=np.uint8)
+                olen = ctypes.c_uint64()
+                check(lib().cucm_stream_compress_next(s, addr + pos, n, out.ctypes.data, out.size,
+                                                      ctypes.byref(olen)), "stream_compress_next")
+                parts.append(out[: olen.value].tobytes())
+                pos += n
+        finally:
+            lib().cucm_stream_end(s)
+        return b"".join(parts)
+
+    def decompress_stream(self, blob, spec_bits=4, records_per_call=1):
+        """Decompresses through the streaming C ABI, `records_per_call` batch
+        records per call."""
+        blob = bytes(blob)
+        hl = ctypes.c_uint64()
+        check(lib().cucm_container_head_len(blob, len(blob), ctypes.byref(hl)), "head_len")
+        N, hp = peek_header(blob)
+        pre = np.empty(max(hp.prelearn_size, 1), dtype=np.uint8)
+        n0 = ctypes.c_uint64()
+        p = Params()
+        s = ctypes.c_void_p()
+        check(lib().cucm_stream_decompress_begin(self.handle, blob, hl.value, spec_bits,
+                                                 ctypes.byref(n0), ctypes.byref(p), pre.ctypes.data,
+                                                 hp.prelearn_size, ctypes.byref(s)),
+              "stream_decompress_begin")
+        parts = [pre[: hp.prelearn_size].tobytes()]
+        try:
+            pos = hl.value
+            per = p.chunk_size * p.batch_chunks
+            while pos < len(blob):
+                start, k = pos, 0
+                while pos < len(blob) and k < records_per_call:
+                    pos += 4 + int.from_bytes(blob[pos:pos + 4], "little")
+                    k += 1
+                recs = blob[start:pos]
+                out = np.empty(max(per * k, 1), dtype=np.uint8)
+                olen = ctypes.c_uint64()
+                check(lib().cucm_stream_decompress_next(s, recs, len(recs), out.ctypes.data,
+                                                        out.size, ctypes.byref(olen)),
+                      "stream_decompress_next")
+                parts.append(out[: olen.value].tobytes())
+        finally:
+            lib().cucm_stream_end(s)
+        return b"".join(parts)
+
     # ---- serving loop (continuous batching) ------------------------------
     def serve_start(self, lanes=0, max_inflight=0):
         """Starts the request loop: `lanes` concurrent launches (0 = 3)."""

@@ -123,6 +123,22 @@ int PinnedBuf::reserve(uint64_t n) {
     if (rc_ != CUCM_OK) return rc_; \
   } while (0)
 
+// The 21-byte container header (cucm_compress_v2.cu:1172-1224, format
+// comment :1420-1443): BE u32 N, BE u16 batch chunks, BE u16 U with the
+// table bits in its low 5 bits (0 = 25), BE u32 S, Pc, P, the flag byte.
+void write_header(uint8_t *h, uint64_t N, const cucm_params &p, uint32_t Pc, uint32_t flag) {
+  put_be32(h, (uint32_t)N);
+  h[4] = (uint8_t)(p.batch_chunks >> 8);
+  h[5] = (uint8_t)p.batch_chunks;
+  const uint32_t uf = p.unit_bits | (p.table_bits == 25 ? 0u : p.table_bits);
+  h[6] = (uint8_t)(uf >> 8);
+  h[7] = (uint8_t)uf;
+  put_be32(h + 8, p.chunk_size);
+  put_be32(h + 12, Pc);
+  put_be32(h + 16, p.prelearn_size);
+  h[20] = (uint8_t)flag;
+}
+
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
@@ -166,7 +182,9 @@ cucm_ctx::~cucm_ctx() {
 
 namespace {
 
-int validate(const cucm_params &p, uint64_t n) {
+}  // namespace
+
+int validate_params(const cucm_params &p, uint64_t n) {
   if (n > 0xffffffffull) return CUCM_E_PARAM;
   if (p.chunk_size == 0 || p.unit_bits == 0 || p.unit_bits % 32 || p.batch_chunks == 0)
     return CUCM_E_PARAM;
@@ -175,6 +193,8 @@ int validate(const cucm_params &p, uint64_t n) {
   if (p.unit_bits > kMaxUnitBits) return CUCM_E_UNSUPPORTED;
   return CUCM_OK;
 }
+
+namespace {
 
 // Waves and launch shape for a persistent launch of `ntasks` chunks.
 struct Shape {
@@ -261,7 +281,7 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
   uint32_t tbmax = 8;
   uint64_t staging_total = 0;
   for (auto &J : jobs) {
-    J.status = validate(J.p, J.n);
+    J.status = validate_params(J.p, J.n);
     J.C = 0;
     J.model = -1;
     J.header.clear();
@@ -272,7 +292,14 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
     // header + prelearn segment
     std::vector<uint8_t> seg;
     uint32_t Pc = 0, flag = 0;
-    if (P > 0) {
+    if (J.ext_model) {
+      if (P) {
+        J.status = CUCM_E_PARAM;
+        continue;
+      }
+      im.hostside.push_back(*J.ext_model);
+      J.model = (int)im.hostside.size() - 1;
+    } else if (P > 0) {
       std::vector<uint8_t> raw(P);
       if (J.host_in)
         std::memcpy(raw.data(), J.host_in, P);
@@ -287,20 +314,10 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
       CU_TRY(upload_model(c, m, im));
       J.model = (int)im.hostside.size() - 1;
     }
-    if (!J.blob_only) {
+    if (!J.blob_only && !J.records_only) {
       J.header.resize(kHeader + seg.size());
-      uint8_t *h = J.header.data();
-      put_be32(h, (uint32_t)J.n);
-      h[4] = (uint8_t)(J.p.batch_chunks >> 8);
-      h[5] = (uint8_t)J.p.batch_chunks;
-      const uint32_t uf = J.p.unit_bits | (J.p.table_bits == 25 ? 0u : J.p.table_bits);
-      h[6] = (uint8_t)(uf >> 8);
-      h[7] = (uint8_t)uf;
-      put_be32(h + 8, J.p.chunk_size);
-      put_be32(h + 12, Pc);
-      put_be32(h + 16, (uint32_t)P);
-      h[20] = (uint8_t)flag;
-      if (!seg.empty()) std::memcpy(h + kHeader, seg.data(), seg.size());
+      write_header(J.header.data(), J.n, J.p, Pc, flag);
+      if (!seg.empty()) std::memcpy(J.header.data() + kHeader, seg.data(), seg.size());
     }
     J.first = (uint32_t)tasks.size();
     const bool pad = !(J.p.flags & CUCM_FLAG_NO_TAIL_PAD);
@@ -524,6 +541,43 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
     DecJob &J = jobs[j];
     J.status = CUCM_OK;
     J.out_len = 0;
+    if (J.records_only) {
+      if (J.body > J.cap) {
+        J.status = CUCM_E_SMALL;
+        continue;
+      }
+      if (J.body > 0 && (J.S == 0 || J.U == 0 || J.batch == 0 || J.U > kMaxUnitBits)) {
+        J.status = CUCM_E_PARAM;
+        continue;
+      }
+      J.out_len = J.body;
+      if (J.body == 0) continue;
+      int model = -1;
+      if (J.ext_model) {
+        im.hostside.push_back(*J.ext_model);
+        model = (int)im.hostside.size() - 1;
+      }
+      ParseJob p{};
+      p.src = (uint64_t)(uintptr_t)J.din;
+      p.len = J.len;
+      p.dst = (uint64_t)(uintptr_t)J.dout;
+      p.body = J.body;
+      p.first_task = ntasks;
+      p.ntasks = (uint32_t)((J.body + J.S - 1) / J.S);
+      p.seg = 0;
+      p.S = J.S;
+      p.U = J.U;
+      p.tb = J.tb;
+      p.batch = J.batch;
+      p.flags = model >= 0 ? kTaskInitModel : 0;
+      p.model = model >= 0 ? (uint32_t)model : 0;
+      ntasks += p.ntasks;
+      Umax = std::max(Umax, J.U);
+      tbmax = std::max(tbmax, J.tb);
+      pj.push_back(p);
+      pj_of.push_back(j);
+      continue;
+    }
     if (J.len < kHeader) {
       J.status = CUCM_E_CORRUPT;
       continue;
@@ -942,6 +996,23 @@ int cucm_decompress_plain_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const
 int cucm_compress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
                        const uint64_t in_len[], const cucm_params params[], uint8_t *const out[],
                        const uint64_t out_cap[], uint64_t out_len[], int32_t status[]) {
+  return compress_host(ctx, n, in, in_len, params, out, out_cap, out_len, status, nullptr);
+}
+
+int cucm_decompress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
+                         const uint64_t in_len[], uint8_t *const out[], const uint64_t out_cap[],
+                         uint64_t out_len[], int32_t status[], uint32_t spec_bits) {
+  return decompress_host(ctx, n, in, in_len, out, out_cap, out_len, status, spec_bits, nullptr);
+}
+
+}  // extern "C"
+
+// Host-buffer compression through the context's pinned staging arena.  With
+// rec set, every job is coded as batch records only (no header) from the
+// initial model rec->ext_model.
+int compress_host(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[], const uint64_t in_len[],
+                  const cucm_params params[], uint8_t *const out[], const uint64_t out_cap[],
+                  uint64_t out_len[], int32_t status[], const EncJob *rec) {
   if (!ctx || (n && (!in || !in_len || !params || !out || !out_cap || !out_len)))
     return CUCM_E_PARAM;
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -970,6 +1041,10 @@ int cucm_compress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
     jobs[i].n = in_len[i];
     jobs[i].p = params[i];
     jobs[i].cap = UINT64_MAX;
+    if (rec) {
+      jobs[i].records_only = true;
+      jobs[i].ext_model = rec->ext_model;
+    }
   }
   CU_TRY(encode_run(ctx, jobs, tm.st));
   std::vector<uint64_t> ooff(n);
@@ -1007,10 +1082,14 @@ int cucm_compress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
   return status ? CUCM_OK : rc;
 }
 
-int cucm_decompress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
-                         const uint64_t in_len[], uint8_t *const out[], const uint64_t out_cap[],
-                         uint64_t out_len[], int32_t status[], uint32_t spec_bits) {
+// Host-buffer decompression through the pinned arena.  With rec set, every
+// input is a run of whole batch records decoding to rec->body bytes with
+// rec's parameters and initial model (n must then be 1).
+int decompress_host(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[], const uint64_t in_len[],
+                    uint8_t *const out[], const uint64_t out_cap[], uint64_t out_len[],
+                    int32_t status[], uint32_t spec_bits, const DecJob *rec) {
   if (!ctx || (n && (!in || !in_len || !out || !out_cap || !out_len))) return CUCM_E_PARAM;
+  if (rec && n != 1) return CUCM_E_PARAM;
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
   CallTimer tm(ctx);
@@ -1019,7 +1098,7 @@ int cucm_decompress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
   for (uint32_t i = 0; i < n; ++i) {
     off[i] = total;
     total += align_up(in_len[i], 256);
-    N[i] = in_len[i] >= kHeader ? get_be32(in[i]) : 0;
+    N[i] = rec ? rec->body : (in_len[i] >= kHeader ? get_be32(in[i]) : 0);
     ooff[i] = ototal;
     ototal += align_up(std::min<uint64_t>(N[i], out_cap[i]), 256);
   }
@@ -1041,6 +1120,15 @@ int cucm_decompress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
     jobs[i].dout = (uint8_t *)ctx->arena_out.p + ooff[i];
     jobs[i].host_out = out[i];
     jobs[i].cap = out_cap[i];
+    if (rec) {
+      jobs[i].records_only = true;
+      jobs[i].body = rec->body;
+      jobs[i].S = rec->S;
+      jobs[i].U = rec->U;
+      jobs[i].tb = rec->tb;
+      jobs[i].batch = rec->batch;
+      jobs[i].ext_model = rec->ext_model;
+    }
   }
   CU_TRY(decode_run(ctx, jobs, spec_bits, tm.st));
   const uint64_t td0 = now_ns();
@@ -1052,10 +1140,10 @@ int cucm_decompress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
   for (uint32_t i = 0; i < n; ++i) {
     if (jobs[i].status == CUCM_OK) {
       // the prelearn bytes were written to host_out directly by the host coder
-      cucm_params hp;
+      cucm_params hp{};
       uint64_t Nn = 0;
-      cucm_peek_header(in[i], in_len[i], &Nn, &hp);
-      const uint64_t P = hp.prelearn_size;
+      if (!rec) cucm_peek_header(in[i], in_len[i], &Nn, &hp);
+      const uint64_t P = rec ? 0 : hp.prelearn_size;
       if (jobs[i].out_len > P)
         std::memcpy(out[i] + P, (uint8_t *)ctx->pin.p + ooff[i] + P, jobs[i].out_len - P);
       out_len[i] = jobs[i].out_len;
@@ -1069,6 +1157,26 @@ int cucm_decompress_many(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[],
   }
   return status ? CUCM_OK : rc;
 }
+
+// Uploads a trained host model once; the caller owns the device copy.
+int upload_init_model(cucm_ctx *c, const HostModel &m, DevBuf &states, DevInitModel &d) {
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  d = DevInitModel{};
+  const uint64_t bytes = 7ull << m.tb;
+  CU_TRY(states.reserve(bytes));
+  HIP_TRY(hipMemcpyAsync(states.p, m.states.data() + ((size_t)1 << m.tb), bytes,
+                         hipMemcpyHostToDevice, c->stream));
+  d.states = (const uint8_t *)states.p;
+  std::memcpy(d.statep, m.statep, sizeof d.statep);
+  std::memcpy(d.w, m.w, sizeof d.w);
+  order0_states(m, d.st0);
+  d.tb = (uint32_t)m.tb;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUCM_OK;
+}
+
+extern "C" {
 
 int cucm_ctx_compress(cucm_ctx *ctx, const uint8_t *in, uint64_t in_len, const cucm_params *p,
                       uint8_t *out, uint64_t out_cap, uint64_t *out_len) {

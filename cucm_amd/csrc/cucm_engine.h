@@ -64,6 +64,10 @@ struct EncJob {
   uint8_t *dout = nullptr;
   uint64_t cap = 0;
   bool blob_only = false;
+  // streaming: batch records only (no header), coded from this already
+  // uploaded initial model instead of a fresh one (p.prelearn_size must be 0)
+  bool records_only = false;
+  const cucm::DevInitModel *ext_model = nullptr;
   int32_t status = 0;
   uint64_t out_len = 0;
   uint32_t C = 0, first = 0;
@@ -80,6 +84,12 @@ struct DecJob {
   uint64_t cap = 0;
   int32_t status = 0;
   uint64_t out_len = 0;
+  // streaming: din holds whole batch records (no header) that decode to
+  // `body` bytes with these parameters and initial model
+  bool records_only = false;
+  uint64_t body = 0;
+  uint32_t S = 0, U = 0, tb = 0, batch = 0;
+  const cucm::DevInitModel *ext_model = nullptr;
 };
 
 int ctx_init(cucm_ctx *c, int device, uint64_t budget);
@@ -87,5 +97,18 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st);
 int encode_assemble(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st);
 int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats &st);
 void serve_shutdown(cucm_ctx *c);
+void write_header(uint8_t *h, uint64_t N, const cucm_params &p, uint32_t Pc, uint32_t flag);
+int validate_params(const cucm_params &p, uint64_t n);
+int compress_host(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[], const uint64_t in_len[],
+                  const cucm_params params[], uint8_t *const out[], const uint64_t out_cap[],
+                  uint64_t out_len[], int32_t status[], const EncJob *rec);
+int decompress_host(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[], const uint64_t in_len[],
+                    uint8_t *const out[], const uint64_t out_cap[], uint64_t out_len[],
+                    int32_t status[], uint32_t spec_bits, const DecJob *rec);
+namespace cucm {
+struct HostModel;
+}
+int upload_init_model(cucm_ctx *c, const cucm::HostModel &m, DevBuf &states,
+                      cucm::DevInitModel &d);
 
 #endif

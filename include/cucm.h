@@ -128,6 +128,38 @@ int cucm_decompress_plain_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const
                                  const uint64_t out_cap[], uint64_t out_len[],
                                  uint32_t spec_bits);
 
+/* ---- streaming container I/O ----------------------------------------- */
+/* The reference's compress_file / decompress (cucm_compress_v2.cu:1151-1250,
+ * cucm_decompress_v2.cu:1184-1248) hold the whole file in host memory; a
+ * stream produces or consumes the same container piece by piece, so the
+ * device footprint is one segment of batch records however large the input
+ * (up to the container's 32-bit size fields: CUCM_E_PARAM beyond them). */
+typedef struct cucm_stream cucm_stream;
+/* Bytes of header + prelearn segment at the front of a container, from its
+ * first 21 bytes. */
+int cucm_container_head_len(const uint8_t *in, uint64_t in_len, uint64_t *head_len);
+/* Starts compressing an input of total_len bytes; prelearn = its first
+ * p->prelearn_size bytes.  Writes the header and prelearn segment to head
+ * (CUCM_E_SMALL, with *head_len set, if head_cap is too small). */
+int cucm_stream_compress_begin(cucm_ctx *ctx, uint64_t total_len, const cucm_params *p,
+                               const uint8_t *prelearn, uint8_t *head, uint64_t head_cap,
+                               uint64_t *head_len, cucm_stream **s);
+/* Codes the next len input bytes after the prelearn bytes -- a multiple of
+ * batch_chunks * chunk_size, or all that remains -- and writes their batch
+ * records (LE u32 length + blob each) to out. */
+int cucm_stream_compress_next(cucm_stream *s, const uint8_t *in, uint64_t len, uint8_t *out,
+                              uint64_t out_cap, uint64_t *out_len);
+/* Starts decompressing: head holds the container's first
+ * cucm_container_head_len bytes.  Returns the original size and parameters
+ * and writes the prelearn bytes to prelearn_out. */
+int cucm_stream_decompress_begin(cucm_ctx *ctx, const uint8_t *head, uint64_t head_len,
+                                 uint32_t spec_bits, uint64_t *orig_len, cucm_params *p,
+                                 uint8_t *prelearn_out, uint64_t prelearn_cap, cucm_stream **s);
+/* Decodes whole batch records (the next ones in container order) to out. */
+int cucm_stream_decompress_next(cucm_stream *s, const uint8_t *records, uint64_t len,
+                                uint8_t *out, uint64_t out_cap, uint64_t *out_len);
+void cucm_stream_end(cucm_stream *s);
+
 /* ---- device memory helpers (for callers without their own allocator) -- */
 int cucm_ctx_malloc(cucm_ctx *ctx, void **dptr, uint64_t bytes);
 int cucm_ctx_free(cucm_ctx *ctx, void *dptr);
