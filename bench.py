@@ -246,6 +246,10 @@ def main():
     ap.add_argument("--chunk-kib", type=int, default=256, help="batch workload: largest chunk")
     ap.add_argument("--plain-files", type=int, default=40,
                     help="batch workload: files in the one-launch-per-file sample")
+    ap.add_argument("--autotune", dest="autotune", action="store_true", default=True,
+                    help="batch workload: tune chunk size, table bits, speculation and waves per "
+                         "workgroup on a sample first (default)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--requests", type=int, default=240, help="serve workload: requests per run")
     ap.add_argument("--lanes", type=int, default=0, help="serve workload: concurrent lanes (0 = 3)")
     args = ap.parse_args()

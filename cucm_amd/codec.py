@@ -125,6 +125,9 @@ class Context:
     def set_max_waves(self, waves):
         check(lib().cucm_ctx_set_max_waves(self.handle, waves))
 
+    def set_waves_per_block(self, waves):
+        check(lib().cucm_ctx_set_waves_per_block(self.handle, waves))
+
     PHASES = ["init", "stage", "hash+gather", "tables+mix", "coder", "mixer+collide", "resolve",
               "statep", "writeback", "dec_paths", "dec_mix+decode", "dec_records"]
 

@@ -217,7 +217,8 @@ Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spe
   if (c->max_waves) waves = std::min<uint64_t>(waves, c->max_waves);
   if (slot_bytes) waves = std::min<uint64_t>(waves, mem_left / slot_bytes);
   if (waves == 0) return s;
-  const uint32_t w = (uint32_t)std::min<uint64_t>(wmax, (waves + c->cus - 1) / c->cus);
+  uint32_t w = (uint32_t)std::min<uint64_t>(wmax, (waves + c->cus - 1) / c->cus);
+  if (c->waves_per_block) w = std::min<uint32_t>(c->waves_per_block, wmax);
   s.wpb = std::max<uint32_t>(w, 1);
   s.blocks = (uint32_t)((waves + s.wpb - 1) / s.wpb);
   if ((uint64_t)s.blocks * s.wpb > waves && s.blocks > 1) {
@@ -822,6 +823,12 @@ void cucm_ctx_destroy(cucm_ctx *ctx) { delete ctx; }
 int cucm_ctx_set_max_waves(cucm_ctx *ctx, uint32_t waves) {
   if (!ctx) return CUCM_E_PARAM;
   ctx->max_waves = waves;
+  return CUCM_OK;
+}
+
+int cucm_ctx_set_waves_per_block(cucm_ctx *ctx, uint32_t waves) {
+  if (!ctx || waves > 8) return CUCM_E_PARAM;
+  ctx->waves_per_block = waves;
   return CUCM_OK;
 }
 

@@ -42,6 +42,7 @@ struct cucm_ctx {
   uint64_t budget = 0;
   uint32_t cus = 0, lds_cu = 0, lds_block = 0;
   uint32_t max_waves = 0;
+  uint32_t waves_per_block = 0;  // 0: as many as LDS allows, spread over the CUs
   bool profile = false;
   uint64_t phase[cucm::kPhases] = {};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

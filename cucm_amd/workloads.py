@@ -2,7 +2,8 @@
 
 batch_workload: 10,000 independent files, lognormal sizes (median 64 KiB,
 sigma 1.5, clipped to [512 B, 64 MiB]), file i generated by the order-2
-Markov source seeded 1000 + i; 2^18 slots per order table.  All files are
+Markov source seeded 1000 + i; 2^18 slots per order table and chunks of at
+most 256 KiB, or the settings the autotuner picks on a sample (--autotune).  All files are
 encoded, then decoded, in one persistent launch each (device-resident
 buffers); the plain path issues one launch per file, as the reference does
 per batch, and is timed on a seeded random sample of the same files because
@@ -38,12 +39,19 @@ def batch_workload(args):
     count = getattr(args, "files", 10000) or 10000
     s_max = getattr(args, "chunk_kib", 256) * 1024
     tb = 18
+    spec = args.spec
     t0 = time.time()
     sizes = datagen.lognormal_sizes(count, 65536, 1.5, 512, 64 * MiB, 3)
     arena, offs = datagen.markov2_files(sizes, seed0=1000)
     total = int(sizes.sum())
     log("generated %d files, %.1f MiB in %.1fs" % (count, total / MiB, time.time() - t0))
     ctx = Context()
+    tuned = None
+    if getattr(args, "autotune", False):
+        from .autotune import autotune_batch
+        tuned = autotune_batch(ctx, arena, offs, sizes, log=log)
+        s_max, tb, spec = tuned["chunk_max"], tuned["table_bits"], tuned["spec_bits"]
+        ctx.set_waves_per_block(tuned["waves_per_block"])
     params = [make_params(pick_chunk_size(int(n), s_max), 128, 100, 0, tb) for n in sizes]
     d_in = ctx.alloc(total)
     d_in.upload(arena)
@@ -58,13 +66,13 @@ def batch_workload(args):
     # batched: one persistent launch each way
     for _ in range(max(args.warmup, 0)):
         cl, st = ctx.compress_many_device(ins, lens, params, cs, caps)
-        ctx.decompress_many_device(cs, cl, outs, lens, args.spec)
+        ctx.decompress_many_device(cs, cl, outs, lens, spec)
     ctx.synchronize()
     a = time.perf_counter()
     cl, st = ctx.compress_many_device(ins, lens, params, cs, caps)
     b = time.perf_counter()
     sc = ctx.stats()
-    dl, dst = ctx.decompress_many_device(cs, cl, outs, lens, args.spec)
+    dl, dst = ctx.decompress_many_device(cs, cl, outs, lens, spec)
     c = time.perf_counter()
     sd = ctx.stats()
     bad = [(i, a, b) for i, (a, b) in enumerate(zip(st, dst)) if a or b]
@@ -86,7 +94,7 @@ def batch_workload(args):
     d = time.perf_counter()
     pl, _ = ctx.compress_many_device(sin, slen, sp, scs, scap, plain=True)
     e = time.perf_counter()
-    pdl, _ = ctx.decompress_many_device(scs, pl, souts, slen, args.spec, plain=True)
+    pdl, _ = ctx.decompress_many_device(scs, pl, souts, slen, spec, plain=True)
     f = time.perf_counter()
     plain_identical = all(pl[k] == cl[i] for k, i in enumerate(sample))
     sample_bytes = sum(slen)
@@ -97,9 +105,12 @@ def batch_workload(args):
         "ms_per_step": 1000 * ((b - a) + (c - b)), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8/int32 fixed-point",
         "data": "synthetic (seeded generator, resident in HBM)",
-        "config": {"workload": "batch: %d lognormal files (median 64 KiB, sigma 1.5), 2^18 slots, "
-                               "chunk = min(file, %d KiB), U=128" % (count, s_max // 1024),
-                   "files": count, "bytes": total},
+        "config": {"workload": "batch: %d lognormal files (median 64 KiB, sigma 1.5), 2^%d slots, "
+                               "chunk = min(file, %d KiB), U=128%s" % (
+                                   count, tb, s_max // 1024, ", autotuned" if tuned else ""),
+                   "files": count, "bytes": total, "table_bits": tb, "chunk_max": s_max,
+                   "spec_bits": spec,
+                   "waves_per_block": tuned["waves_per_block"] if tuned else 0},
         "batched_compress_MBps": total / 1e6 / (b - a),
         "batched_decompress_MBps": total / 1e6 / (c - b),
         "batched_compress_kernel_ms": sc["codec_ms"],
@@ -111,6 +122,13 @@ def batch_workload(args):
         "ratio": total / comp_total, "lossless": lossless, "tasks": sc["tasks"],
         "waves": sc["waves"],
     }
+    if tuned:
+        res["autotune"] = {k: tuned[k] for k in ("chunk_max", "table_bits", "spec_bits",
+                                                 "waves_per_block", "sample_files",
+                                                 "sample_bytes", "ratio_tol", "seconds")}
+        res["autotune"]["trials"] = [{k: t[k] for k in ("chunk_max", "table_bits", "spec_bits",
+                                                        "waves_per_block", "ratio", "score")}
+                                     for t in tuned["stream_trials"] + tuned["launch_trials"]]
     res["batched_vs_plain_compress"] = res["batched_compress_MBps"] / res["plain_compress_MBps"]
     res["batched_vs_plain_decompress"] = res["batched_decompress_MBps"] / res["plain_decompress_MBps"]
     return res

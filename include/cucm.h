@@ -195,6 +195,10 @@ int cucm_ctx_phase_cycles(cucm_ctx *ctx, uint64_t cycles[CUCM_PHASES]);
 
 /* Upper bound on concurrently resident worker waves (0 = automatic). */
 int cucm_ctx_set_max_waves(cucm_ctx *ctx, uint32_t waves);
+/* Waves per workgroup, 1..8 (0 = automatic): the waves of a workgroup share
+ * one LDS copy of the lookup tables; fewer per workgroup spreads a small
+ * launch over more CUs.  Never changes the output. */
+int cucm_ctx_set_waves_per_block(cucm_ctx *ctx, uint32_t waves);
 
 /* ---- measurement probes (device 0 / current device) ------------------- */
 /* Random one-byte gather microbenchmark over `bytes` (a power of two) of
