@@ -951,6 +951,7 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
       const uint32_t cur = buf;
       buf ^= 1u;
       if (rb + D < nbits) paths_issue<NM, P>(M, slot, tb, mask, (rb + D) & 7u, partial, hist, buf, lane, g);
+      clk.mark(1);
       // record the taken paths: slot and unit-start state of every order of
       // every decided bit, and (pr, y), cxt per bit
       {
