@@ -26,7 +26,7 @@ EXPORTS = [
     "cucm_ctx_set_phase_profile", "cucm_ctx_phase_cycles",
     "cucm_serve_start", "cucm_serve_stop", "cucm_submit", "cucm_poll", "cucm_serve_launches",
     "cucm_gather_probe", "cucm_copy_probe",
-    "cucm_ctx_set_waves_per_block", "cucm_container_head_len", "cucm_stream_compress_begin", "cucm_stream_compress_next",
+    "cucm_ctx_set_waves_per_block", "cucm_ctx_set_team", "cucm_container_head_len", "cucm_stream_compress_begin", "cucm_stream_compress_next",
     "cucm_stream_decompress_begin", "cucm_stream_decompress_next", "cucm_stream_end",
 ]
 
@@ -146,6 +146,7 @@ def lib():
         L.cucm_copy_probe.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         L.cucm_container_head_len.argtypes = [vp, u64, P(u64)]
         L.cucm_ctx_set_waves_per_block.argtypes = [vp, u32]
+        L.cucm_ctx_set_team.argtypes = [vp, u32]
         L.cucm_stream_compress_begin.argtypes = [vp, u64, P(Params), vp, vp, u64, P(u64), P(vp)]
         L.cucm_stream_compress_next.argtypes = [vp, vp, u64, vp, u64, P(u64)]
         L.cucm_stream_decompress_begin.argtypes = [vp, vp, u64, u32, P(u64), P(Params), vp, u64,

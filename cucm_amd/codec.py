@@ -128,6 +128,10 @@ class Context:
     def set_waves_per_block(self, waves):
         check(lib().cucm_ctx_set_waves_per_block(self.handle, waves))
 
+    def set_team(self, mode):
+        """Decoder team execution: 0 automatic, 1 never, 2 always."""
+        check(lib().cucm_ctx_set_team(self.handle, mode))
+
     PHASES = ["init", "stage", "hash+gather", "tables+mix", "coder", "mixer+collide", "resolve",
               "statep", "writeback", "dec_paths", "dec_mix+decode", "dec_records"]
 

@@ -196,15 +196,46 @@ int validate_params(const cucm_params &p, uint64_t n) {
 
 namespace {
 
-// Waves and launch shape for a persistent launch of `ntasks` chunks.
+// Waves and launch shape for a persistent launch of `ntasks` chunks: either
+// single-wave chunks (wpb waves per workgroup sharing the tables, one model
+// slot per wave) or teams (one chunk and one model slot per workgroup of
+// kTeamWaves waves).
 struct Shape {
   uint32_t blocks = 0, wpb = 0;
+  bool team = false;
   uint32_t waves() const { return blocks * wpb; }
+  uint32_t slots() const { return team ? blocks : blocks * wpb; }
 };
+
+// Team workgroups the launch can hold at once (0 if a team does not fit).
+uint64_t team_capacity(const cucm_ctx *c, uint32_t Umax, uint32_t spec) {
+  const uint32_t lds = team_codec_lds_bytes(Umax, spec);
+  if (lds > c->lds_block) return 0;
+  return (uint64_t)c->cus * std::max<uint32_t>(1, c->lds_cu / lds);
+}
+
+// Teams run when forced on, or (automatic) when every chunk of the launch
+// can be resident as a team at once: then a chunk's latency, not the chip's
+// memory traffic, bounds the launch, and the team's extra waves shorten it.
+bool use_team(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, bool dec) {
+  if (c->team_mode == 1 || !team_supported(dec, spec)) return false;
+  const uint64_t cap = team_capacity(c, Umax, spec);
+  if (cap == 0) return false;
+  return c->team_mode == 2 || ntasks <= cap;
+}
 
 Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, bool dec,
                  uint64_t slot_bytes, uint64_t mem_left) {
   Shape s;
+  if (use_team(c, ntasks, Umax, spec, dec)) {
+    uint64_t blocks = std::min<uint64_t>(ntasks, team_capacity(c, Umax, spec));
+    if (c->max_waves) blocks = std::min<uint64_t>(blocks, std::max<uint32_t>(1, c->max_waves / kTeamWaves));
+    if (slot_bytes) blocks = std::min<uint64_t>(blocks, mem_left / slot_bytes);
+    s.team = true;
+    s.wpb = kTeamWaves;
+    s.blocks = (uint32_t)blocks;
+    return s;
+  }
   uint32_t wmax = 1;
   for (uint32_t w = 8; w >= 1; --w)
     if (codec_lds_bytes(Umax, spec, dec, w) <= c->lds_block) {
@@ -367,10 +398,10 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
                         c->arena_out.cap;
   const uint64_t mem_left = c->budget > used ? c->budget - used : 0;
   Shape sh = pick_shape(c, ntasks, Umax, 0, false, slot_bytes, mem_left);
-  if (sh.waves() == 0) return CUCM_E_BUDGET;
-  CU_TRY(c->pool.reserve(slot_bytes * sh.waves()));
+  if (sh.slots() == 0) return CUCM_E_BUDGET;
+  CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
   st.waves = std::max<uint64_t>(st.waves, sh.waves());
-  st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.waves());
+  st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
   a.ntasks = ntasks;
@@ -384,7 +415,7 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
   CU_TRY(prof_arm(c, a));
   HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  HIP_TRY(launch_codec(false, a, Umax, sh.blocks, sh.wpb, c->stream));
+  HIP_TRY(launch_codec(false, a, Umax, sh.blocks, sh.wpb, sh.team, c->stream));
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   st.launches += 1;
   CU_TRY(prof_collect(c, a));
@@ -423,9 +454,9 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
     b.ntasks = (uint32_t)rt.size();
     b.result = (uint64_t *)c->results2.p;
     Shape s2 = pick_shape(c, rt.size(), Umax, 0, false, slot_bytes, mem_left);
-    if (s2.waves() == 0) return CUCM_E_BUDGET;
+    if (s2.slots() == 0) return CUCM_E_BUDGET;
     HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
-    HIP_TRY(launch_codec(false, b, Umax, s2.blocks, s2.wpb, c->stream));
+    HIP_TRY(launch_codec(false, b, Umax, s2.blocks, s2.wpb, s2.team, c->stream));
     st.launches += 1;
     std::vector<uint64_t> r2(rt.size());
     HIP_TRY(hipMemcpyAsync(r2.data(), c->results2.p, 8 * rt.size(), hipMemcpyDeviceToHost,
@@ -675,10 +706,10 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
                         c->arena_out.cap;
   const uint64_t mem_left = c->budget > used ? c->budget - used : 0;
   Shape sh = pick_shape(c, ntasks, Umax, spec, true, slot_bytes, mem_left);
-  if (sh.waves() == 0) return CUCM_E_BUDGET;
-  CU_TRY(c->pool.reserve(slot_bytes * sh.waves()));
+  if (sh.slots() == 0) return CUCM_E_BUDGET;
+  CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
   st.waves = std::max<uint64_t>(st.waves, sh.waves());
-  st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.waves());
+  st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
   a.ntasks = ntasks;
@@ -692,7 +723,7 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   CU_TRY(prof_arm(c, a));
   HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  HIP_TRY(launch_codec(true, a, Umax, sh.blocks, sh.wpb, c->stream));
+  HIP_TRY(launch_codec(true, a, Umax, sh.blocks, sh.wpb, sh.team, c->stream));
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   st.launches += 1;
   CU_TRY(prof_collect(c, a));
@@ -823,6 +854,12 @@ void cucm_ctx_destroy(cucm_ctx *ctx) { delete ctx; }
 int cucm_ctx_set_max_waves(cucm_ctx *ctx, uint32_t waves) {
   if (!ctx) return CUCM_E_PARAM;
   ctx->max_waves = waves;
+  return CUCM_OK;
+}
+
+int cucm_ctx_set_team(cucm_ctx *ctx, uint32_t mode) {
+  if (!ctx || mode > 2) return CUCM_E_PARAM;
+  ctx->team_mode = mode;
   return CUCM_OK;
 }
 

@@ -43,6 +43,7 @@ struct cucm_ctx {
   uint32_t cus = 0, lds_cu = 0, lds_block = 0;
   uint32_t max_waves = 0;
   uint32_t waves_per_block = 0;  // 0: as many as LDS allows, spread over the CUs
+  uint32_t team_mode = 0;        // 0: teams when every chunk fits at once, 1: never, 2: always
   bool profile = false;
   uint64_t phase[cucm::kPhases] = {};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -2,7 +2,7 @@
 // codec: the persistent codec kernel (its per-wave code is in cucm_wave.h),
 // container assembly, device-side container parsing, and the host launchers.
 #define CUCM_WITH_LAUNCHERS
-#include "cucm_wave.h"
+#include "cucm_team.h"
 
 namespace cucm {
 
@@ -18,13 +18,18 @@ __device__ __forceinline__ void stage_tables(const DevTables *g, LDS DevTables *
   __syncthreads();
 }
 
+// Largest unit of a launch rounded up to what its NJ covers: the LDS layout
+// is laid out for it, so every LDS offset is a compile-time constant.
+CUCM_HD uint32_t unit_capacity(uint32_t Umax) { return Umax <= 64 ? 64u : (Umax <= 128 ? 128u : 256u); }
+
 template <bool kDecode, int NJ, int D>
-__global__ void __launch_bounds__(512) codec_kernel(CodecArgs a, uint32_t Umax) {
+__global__ void __launch_bounds__(512) codec_kernel(CodecArgs a) {
+  constexpr uint32_t Umax = 64u * NJ;
   extern __shared__ __align__(16) uint8_t lds[];
   LDS DevTables *T = (LDS DevTables *)lds;
   stage_tables(a.tabs, T);
   const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t P = kDecode ? (1u << a.spec_bits) - 1u : 0u;
+  constexpr uint32_t P = kDecode ? (1u << D) - 1u : 0u;
   const uint32_t wbytes = wave_lds_bytes(Umax, P, kDecode);
   LDS uint8_t *wbase = (LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * wbytes;
   const WaveMem M = carve(wbase, Umax, P, kDecode);
@@ -40,6 +45,31 @@ __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a, uint32_t Umax) 
       decode_task<NJ, D>(a, tk, t, M, T, slot, Umax, lane);
     else
       encode_task<NJ>(a, tk, t, M, T, slot, Umax, lane);
+  }
+}
+
+// The team decoder: one chunk per workgroup of kTeamWaves waves, each
+// workgroup owning one model slot and pulling chunk tasks from the queue.
+template <int NJ, int D>
+__global__ void __launch_bounds__(256) decode_team_kernel(CodecArgs a) {
+  constexpr uint32_t Umax = 64u * NJ;
+  extern __shared__ __align__(16) uint8_t lds[];
+  LDS DevTables *T = (LDS DevTables *)lds;
+  stage_tables(a.tabs, T);
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  LDS uint8_t *tbase = (LDS uint8_t *)lds + align16(sizeof(DevTables));
+  const TeamMem X = carve_team(tbase, Umax, D, wid);
+  GLB uint8_t *slot = (GLB uint8_t *)a.pool + (uint64_t)blockIdx.x * a.slot_bytes;
+  uint32_t rid = 0;  // running round id (identical in every wave)
+  for (;;) {
+    if (threadIdx.x == 0) X.ctl[0] = atomicAdd(a.counter, 1u);
+    team_sync();
+    const uint32_t t = __builtin_amdgcn_readfirstlane(X.ctl[0]);
+    team_sync();
+    if (t >= a.ntasks) break;
+    const Task tk = a.tasks[t];
+    decode_team<NJ, D>(a, tk, t, X, T, slot, Umax, wid, lane, rid);
   }
 }
 
@@ -191,7 +221,38 @@ __global__ void build_decode_tasks_kernel(const ParseJob *jobs, uint32_t n, Task
 
 uint32_t codec_lds_bytes(uint32_t Umax, uint32_t spec_bits, bool decode, uint32_t waves) {
   const uint32_t P = decode ? (1u << spec_bits) - 1u : 0u;
-  return align16(sizeof(DevTables)) + waves * wave_lds_bytes(Umax, P, decode);
+  return align16(sizeof(DevTables)) + waves * wave_lds_bytes(unit_capacity(Umax), P, decode);
+}
+
+uint32_t team_codec_lds_bytes(uint32_t Umax, uint32_t spec_bits) {
+  return align16(sizeof(DevTables)) + team_lds_bytes(unit_capacity(Umax), spec_bits);
+}
+
+bool team_supported(bool decode, uint32_t spec_bits) {
+  return decode && spec_bits >= 1 && spec_bits <= kTeamMaxSpec && (8 % spec_bits) == 0;
+}
+
+template <int NJ, int D>
+static hipError_t launch_team_one(const CodecArgs &a, uint32_t Umax, uint32_t blocks,
+                                  hipStream_t stream) {
+  const uint32_t lds = team_codec_lds_bytes(Umax, D);
+  hipError_t e = hipFuncSetAttribute((const void *)decode_team_kernel<NJ, D>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((decode_team_kernel<NJ, D>), dim3(blocks), dim3(64 * kTeamWaves), lds,
+                     stream, a);
+  return hipGetLastError();
+}
+
+template <int NJ>
+static hipError_t launch_team(const CodecArgs &a, uint32_t Umax, uint32_t blocks,
+                              hipStream_t stream) {
+  switch (a.spec_bits) {
+    case 1: return launch_team_one<NJ, 1>(a, Umax, blocks, stream);
+    case 2: return launch_team_one<NJ, 2>(a, Umax, blocks, stream);
+    case 4: return launch_team_one<NJ, 4>(a, Umax, blocks, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <bool kDecode, int NJ, int D>
@@ -200,8 +261,7 @@ static hipError_t launch_one(const CodecArgs &a, uint32_t Umax, uint32_t blocks,
   hipError_t e = hipFuncSetAttribute((const void *)codec_kernel<kDecode, NJ, D>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((codec_kernel<kDecode, NJ, D>), dim3(blocks), dim3(64 * wpb), lds, stream, a,
-                     Umax);
+  hipLaunchKernelGGL((codec_kernel<kDecode, NJ, D>), dim3(blocks), dim3(64 * wpb), lds, stream, a);
   return hipGetLastError();
 }
 
@@ -218,9 +278,15 @@ static hipError_t launch_decode(const CodecArgs &a, uint32_t Umax, uint32_t bloc
 }
 
 hipError_t launch_codec(bool decode, const CodecArgs &a, uint32_t Umax, uint32_t blocks,
-                        uint32_t waves_per_block, hipStream_t stream) {
+                        uint32_t waves_per_block, bool team, hipStream_t stream) {
   const uint32_t lds = codec_lds_bytes(Umax, a.spec_bits, decode, waves_per_block);
   const int nj = Umax <= 64 ? 1 : (Umax <= 128 ? 2 : 4);
+  if (team) {
+    if (!team_supported(decode, a.spec_bits)) return hipErrorInvalidValue;
+    if (nj == 1) return launch_team<1>(a, Umax, blocks, stream);
+    if (nj == 2) return launch_team<2>(a, Umax, blocks, stream);
+    return launch_team<4>(a, Umax, blocks, stream);
+  }
   if (decode) {
     if (nj == 1) return launch_decode<1>(a, Umax, blocks, waves_per_block, lds, stream);
     if (nj == 2) return launch_decode<2>(a, Umax, blocks, waves_per_block, lds, stream);

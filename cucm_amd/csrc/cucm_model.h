@@ -11,6 +11,7 @@ namespace cucm {
 constexpr int kOrders = 8;        // context orders 0..7 (cucm_compress_v2.cu:552)
 constexpr int kMixCtx = 80;       // mixer weight sets (cucm_compress_v2.cu:554)
 constexpr int kMaxUnitBits = 256; // largest unit the LDS staging is laid out for
+constexpr uint32_t kTeamWaves = 4; // waves of a team workgroup (one chunk per workgroup)
 constexpr uint32_t kFnvBasis = 2166136261u;
 constexpr uint32_t kFnvPrime = 16777619u;
 
@@ -146,8 +147,12 @@ struct ParseJob {
 #include <hip/hip_runtime_api.h>
 namespace cucm {
 uint32_t codec_lds_bytes(uint32_t Umax, uint32_t spec_bits, bool decode, uint32_t waves);
+// LDS of one team workgroup (one chunk per workgroup of several waves) and
+// whether a launch can run as teams.
+uint32_t team_codec_lds_bytes(uint32_t Umax, uint32_t spec_bits);
+bool team_supported(bool decode, uint32_t spec_bits);
 hipError_t launch_codec(bool decode, const CodecArgs &a, uint32_t Umax, uint32_t blocks,
-                        uint32_t waves_per_block, hipStream_t stream);
+                        uint32_t waves_per_block, bool team, hipStream_t stream);
 hipError_t launch_read_headers(const uint64_t *src, const uint64_t *len, uint32_t n, uint8_t *hdr,
                                hipStream_t stream);
 hipError_t launch_build_decode_tasks(const ParseJob *jobs, uint32_t n, Task *tasks,

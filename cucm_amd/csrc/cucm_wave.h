@@ -68,6 +68,29 @@ CUCM_DEV void vm_drain() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
+// Workgroup barrier of a multi-wave team: LDS and global writes before it
+// (global stores completed) are seen by every wave after it.
+CUCM_DEV void team_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  wait_vm();
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+// Workgroup barrier that orders LDS only: the waves' outstanding global loads
+// (a leader's gathers in flight) stay in flight across it.
+CUCM_DEV void team_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+// An LDS word published by one wave and polled by another: the writes made
+// before the store are seen by a wave whose load returns the stored value.
+CUCM_DEV void flag_publish(LDS uint32_t *p, uint32_t v) {
+  __hip_atomic_store((uint32_t *)p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+CUCM_DEV uint32_t flag_read(const LDS uint32_t *p) {
+  return __hip_atomic_load((uint32_t *)p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 CUCM_DEV uint32_t lds_cas(LDS uint32_t *p, uint32_t cmp, uint32_t v) {
   return atomicCAS((uint32_t *)p, cmp, v);
 }
@@ -111,12 +134,13 @@ struct WaveMem {
   LDS uint32_t *dummy;  // [64] scratch target of predicated-off LDS operations
 };
 
-// Byte offsets of the per-wave LDS arrays (16-byte aligned each).
+// Byte offsets of the per-wave LDS arrays (16-byte aligned each); the
+// decoder keeps nbuf rounds of path slots and states.
 struct WaveLayout {
   uint32_t stp, w, tkey, pslot, tval, ridx, rprm, st0, rg, rcxt, pg, ub, dummy, total;
 };
 
-CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec) {
+CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec, uint32_t nbuf = 2) {
   const uint32_t TS = table_size(Umax);
   const uint32_t Pd = dec ? P : 0u;
   WaveLayout l;
@@ -127,14 +151,14 @@ CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec) {
   CUCM_TAKE(stp, 8 * 256 * 4)
   CUCM_TAKE(w, kMixCtx * kOrders * 4)
   CUCM_TAKE(tkey, 7 * TS * 4)
-  CUCM_TAKE(pslot, 2 * Pd * 8 * 4)
+  CUCM_TAKE(pslot, nbuf * Pd * 8 * 4)
   CUCM_TAKE(tval, 7 * TS * 2)
   CUCM_TAKE(ridx, 8 * Umax * 2)
   CUCM_TAKE(rprm, Umax * 2)
   CUCM_TAKE(st0, 256)
   CUCM_TAKE(rg, 8 * Umax)
   CUCM_TAKE(rcxt, Umax)
-  CUCM_TAKE(pg, 2 * Pd * 8)
+  CUCM_TAKE(pg, nbuf * Pd * 8)
   CUCM_TAKE(ub, 8 + Umax / 8 + 8)
   CUCM_TAKE(dummy, 64 * 4)
 #undef CUCM_TAKE
@@ -146,8 +170,9 @@ CUCM_HD uint32_t wave_lds_bytes(uint32_t Umax, uint32_t P, bool dec) {
   return wave_layout(Umax, P, dec).total;
 }
 
-CUCM_DEV WaveMem carve(LDS uint8_t *base, uint32_t Umax, uint32_t P, bool dec) {
-  const WaveLayout l = wave_layout(Umax, P, dec);
+CUCM_DEV WaveMem carve(LDS uint8_t *base, uint32_t Umax, uint32_t P, bool dec,
+                       uint32_t nbuf = 2) {
+  const WaveLayout l = wave_layout(Umax, P, dec, nbuf);
   WaveMem m;
   m.stp = (LDS uint32_t *)(base + l.stp);
   m.w = (LDS int32_t *)(base + l.w);
@@ -197,30 +222,44 @@ struct PhaseClock {
 // Model primitives (device)
 // ---------------------------------------------------------------------------
 
-// Mixer::p (cucm_compress_v2.cu:636-644) plus the pr bump (:787-788).
-CUCM_DEV uint32_t mix_predict(const LDS int32_t *wrow, const int32_t smp[8],
-                                                const LDS uint16_t *squash) {
-  int64_t dot = 0;
+// floor(sum_k w[k] * smp[k] / 2^16), the mixer's dot product
+// (cucm_compress_v2.cu:636-640), for stretch outputs |smp[k]| <= 2047.  Each
+// weight splits into a signed high and an unsigned low 16-bit half, so every
+// product is a 24-bit multiply and both partial sums fit in 32 bits
+// (|A| <= 8 * 2^15 * 2047, |B| <= 8 * (2^16 - 1) * 2047 < 2^31); then
+// (A * 2^16 + B) >> 16 = A + (B >> 16) with arithmetic shifts, exactly.
+CUCM_DEV int32_t mix_dot(const int32_t w[8], const int32_t smp[8]) {
+  int32_t A = 0, B = 0;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) dot += (int64_t)wrow[k] * (int64_t)smp[k];
-  dot >>= 16;
-  const int idx = dot < -2047 ? -2047 : (dot > 2048 ? 2048 : (int)dot);
+  for (int k = 0; k < 8; ++k) {
+    A += (w[k] >> 16) * smp[k];
+    B += (int32_t)((uint32_t)w[k] & 0xffffu) * smp[k];
+  }
+  return A + (B >> 16);
+}
+
+// squash of the clamped dot product plus the pr bump (cucm_compress_v2.cu:
+// 641-644, 787-788).
+CUCM_DEV uint32_t mix_squash(int32_t dot, const LDS uint16_t *squash) {
+  const int idx = dot < -2047 ? -2047 : (dot > 2048 ? 2048 : dot);
   uint32_t pr = squash[idx + 2047];
   if (pr < 2048) ++pr;
   return pr;
 }
 
+// Mixer::p (cucm_compress_v2.cu:636-644) plus the pr bump (:787-788).
+CUCM_DEV uint32_t mix_predict(const LDS int32_t *wrow, const int32_t smp[8],
+                                                const LDS uint16_t *squash) {
+  int32_t w[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) w[k] = wrow[k];
+  return mix_squash(mix_dot(w, smp), squash);
+}
+
 // Mixer::p with the weight row already in registers.
 CUCM_DEV uint32_t mix_weights(const int32_t w[8], const int32_t smp[8],
                                                 const LDS uint16_t *squash) {
-  int64_t dot = 0;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) dot += (int64_t)w[k] * (int64_t)smp[k];
-  dot >>= 16;
-  const int idx = dot < -2047 ? -2047 : (dot > 2048 ? 2048 : (int)dot);
-  uint32_t pr = squash[idx + 2047];
-  if (pr < 2048) ++pr;
-  return pr;
+  return mix_squash(mix_dot(w, smp), squash);
 }
 
 // Carry-less coder interval split (cucm_compress_v2.cu:855-857):

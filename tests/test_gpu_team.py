@@ -1,0 +1,96 @@
+"""The team decoder (one chunk per workgroup of four waves, cucm_team.h) on
+the GPU: lossless on every configuration and speculation width it supports,
+with a prelearned model, with more chunks than resident teams (workgroups
+reuse their LDS and model slot from chunk to chunk), over many containers in
+one launch, and chosen automatically for a launch whose chunks all fit."""
+import numpy as np
+import pytest
+
+import cucm_amd
+from cucm_amd import datagen, make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(kind, n, seed):
+    if kind == "markov2":
+        return datagen.markov2(n, seed).tobytes()
+    if kind == "mixed":
+        return datagen.mixed(n, seed).tobytes()
+    if kind == "structured":
+        return datagen.structured(n, seed).tobytes()
+    if kind == "random":
+        return np.random.default_rng(seed).integers(0, 256, n, dtype=np.uint8).tobytes()
+    return bytes(n)
+
+
+@pytest.fixture(scope="module")
+def tctx():
+    c = cucm_amd.Context()
+    yield c
+    c.close()
+
+
+CASES = [
+    # kind, N, S, U, batch, P, tb, spec
+    ("markov2", 70000, 16384, 128, 100, 0, 18, 4),
+    ("mixed", 40000, 4096, 64, 4, 0, 16, 2),
+    ("random", 9000, 4096, 96, 100, 0, 14, 1),
+    ("zeros", 20000, 2048, 256, 5, 0, 18, 4),
+    ("structured", 30000, 8192, 128, 2, 0, 20, 4),
+    ("markov2", 10, 4096, 128, 100, 0, 18, 4),
+    ("markov2", 1, 1, 32, 100, 0, 8, 2),
+    ("markov2", 60000, 8192, 128, 3, 7000, 16, 4),  # prelearned initial model
+    ("mixed", 3000, 3000, 128, 1, 0, 25, 4),        # reference table size
+]
+
+
+@pytest.mark.parametrize("kind,N,S,U,batch,P,tb,spec", CASES)
+def test_team_decoder_lossless(tctx, oracle, kind, N, S, U, batch, P, tb, spec):
+    data = _data(kind, N, 11)
+    blob = oracle.compress(data, S, U, batch, P, tb)
+    tctx.set_team(2)
+    try:
+        assert tctx.decompress(blob, spec) == data
+    finally:
+        tctx.set_team(0)
+
+
+def test_team_workgroups_cycle_through_many_chunks(tctx, oracle):
+    data = _data("markov2", 300_000, 5)
+    blob = oracle.compress(data, 4096, 128, 100, 0, 16)
+    tctx.set_team(2)
+    tctx.set_max_waves(8)  # two teams for 74 chunks
+    try:
+        assert tctx.decompress(blob, 4) == data
+        assert tctx.stats()["waves"] == 8
+    finally:
+        tctx.set_team(0)
+        tctx.set_max_waves(0)
+
+
+def test_team_many_containers_one_launch(tctx, oracle):
+    datas = [_data("mixed", n, 100 + i) for i, n in enumerate((1, 700, 5000, 20000, 65536))]
+    blobs = [oracle.compress(d, 4096, 64, 2, 0, 14) for d in datas]
+    tctx.set_team(2)
+    try:
+        assert tctx.decompress_many(blobs, 2) == datas
+    finally:
+        tctx.set_team(0)
+
+
+def test_auto_mode_runs_few_chunks_as_teams(tctx, oracle):
+    data = _data("structured", 40000, 9)
+    p = make_params(8192, 128, 100, 0, 16)
+    blob = tctx.compress(data, p)
+    assert blob == oracle.compress(data, 8192, 128, 100, 0, 16)
+    for mode, spec in ((0, 4), (1, 4), (0, 8)):
+        tctx.set_team(mode)
+        try:
+            assert tctx.decompress(blob, spec) == data
+            # 5 chunks: teams of 4 waves in automatic mode with 1, 2 or 4
+            # speculation bits; one wave per chunk otherwise
+            want = 20 if (mode == 0 and spec <= 4) else 5
+            assert tctx.stats()["waves"] == want
+        finally:
+            tctx.set_team(0)

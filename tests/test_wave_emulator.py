@@ -79,3 +79,36 @@ def test_emulated_wave_code_matches_oracle(emu, oracle, tmp_path, kind, N, S, U,
                        capture_output=True, text=True, env=env)
     assert r.returncode == 0, r.stderr[-4000:]
     assert dec.read_bytes() == data
+
+
+TEAM_CASES = [
+    # (kind, N, S, U, tb, spec_bits, seeds)
+    ("markov2", 6000, 2048, 128, 12, 4, (1, 2, 3)),
+    ("markov2", 4100, 1024, 64, 10, 1, (4,)),
+    ("structured", 3000, 1024, 96, 11, 2, (5, 6)),
+    ("mixed", 5000, 2048, 256, 14, 4, (7,)),
+    ("random", 1500, 512, 32, 9, 4, (8, 9)),
+    ("markov2", 40, 4096, 128, 8, 4, (10,)),
+]
+
+
+@pytest.mark.parametrize("kind,N,S,U,tb,spec,seeds", TEAM_CASES)
+def test_emulated_team_decoder_is_lossless(emu, oracle, tmp_path, kind, N, S, U, tb, spec, seeds):
+    """The team decoder (cucm_team.h): every chunk in turn on one emulated
+    workgroup of four waves sharing one LDS, the waves stepped in orders drawn
+    from several seeds, decodes the oracle's payloads losslessly."""
+    data = _data(kind, N)
+    C = (N + S - 1) // S
+    blob = b""
+    for c in range(C):
+        p = oracle.encode_chunk(data[c * S:(c + 1) * S], S, U, oracle.Model(tb),
+                                tail_pad=c == C - 1)
+        blob += struct.pack("<I", len(p)) + p
+    enc, dec = tmp_path / "enc", tmp_path / "dec"
+    enc.write_bytes(blob)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    for seed in seeds:
+        r = subprocess.run([emu, "tdec", str(enc), str(N), str(S), str(U), str(tb), str(spec),
+                            str(seed), str(dec)], capture_output=True, text=True, env=env)
+        assert r.returncode == 0, r.stderr[-4000:]
+        assert dec.read_bytes() == data, "seed %d" % seed

@@ -6,10 +6,14 @@
 //       output: per chunk, a u32 LE payload length and the payload
 //   wave_emu dec <payloads> <N> <S> <U> <tb> <spec_bits> <output>
 //       payloads in the format enc writes; output: the N decoded bytes
+//   wave_emu tdec <payloads> <N> <S> <U> <tb> <spec_bits> <seed> <output>
+//       the same through the team decoder (cucm_team.h): every chunk in turn
+//       on one workgroup of kTeamWaves waves and one LDS, waves stepped in
+//       an order drawn from seed
 #define CUCM_WAVE_EMULATION 1
 #include "wave_emulation.h"
 // (the device code follows the emulation shims)
-#include "cucm_wave.h"
+#include "cucm_team.h"
 #include "cucm_host_model.h"
 
 #include <cstring>
@@ -130,6 +134,72 @@ static void decode_d(int spec, const std::vector<uint8_t> &pay, uint8_t *out, ui
   }
 }
 
+// All chunks through one team, as the persistent team kernel runs them.
+template <int NJ, int D>
+static void decode_team_all(const std::vector<uint8_t> &in, uint8_t *out, uint64_t N, uint32_t S,
+                            uint32_t U, uint32_t tb, uint64_t seed) {
+  const uint64_t C = (N + S - 1) / S;
+  std::vector<Task> tasks(C);
+  std::vector<uint8_t *> srcs(C), dsts(C);
+  size_t pos = 0;
+  for (uint64_t c = 0; c < C; ++c) {
+    const uint32_t L = (uint32_t)std::min<uint64_t>(S, N - c * S);
+    uint32_t n = 0;
+    for (int i = 0; i < 4; ++i) n |= (uint32_t)in[pos + i] << (8 * i);
+    pos += 4;
+    srcs[c] = (uint8_t *)malloc(n ? n : 1);
+    memcpy(srcs[c], in.data() + pos, n);
+    pos += n;
+    dsts[c] = (uint8_t *)malloc(L ? L : 1);
+    Task &tk = tasks[c];
+    tk = Task{};
+    tk.src = (uint64_t)(uintptr_t)srcs[c];
+    tk.dst = (uint64_t)(uintptr_t)dsts[c];
+    tk.L = L;
+    tk.S = S;
+    tk.aux = n;
+    tk.U = (uint16_t)U;
+    tk.tb = (uint8_t)tb;
+  }
+  const size_t tab = align16(sizeof(DevTables));
+  const size_t bytes = tab + team_lds_bytes(U, D);
+  uint8_t *base = (uint8_t *)aligned_alloc(16, (bytes + 15) & ~(size_t)15);
+  memset(base, 0xa5, bytes);
+  memcpy(base, &host_tables(), sizeof(DevTables));
+  const DevTables *T = (const DevTables *)base;
+  uint8_t *slot = (uint8_t *)malloc((size_t)7 << tb);
+  std::vector<uint64_t> result(C, 0);
+  CodecArgs a{};
+  a.tasks = tasks.data();
+  a.ntasks = (uint32_t)C;
+  a.spec_bits = D;
+  a.result = result.data();
+  emu::run_team(kTeamWaves, seed, [&](uint32_t wid, uint32_t lane) {
+    const TeamMem X = carve_team(base + tab, U, D, wid);
+    uint32_t rid = 0;
+    for (uint64_t c = 0; c < C; ++c)
+      decode_team<NJ, D>(a, tasks[c], (uint32_t)c, X, T, slot, U, wid, lane, rid);
+  });
+  for (uint64_t c = 0; c < C; ++c) {
+    memcpy(out + c * S, dsts[c], tasks[c].L);
+    free(srcs[c]);
+    free(dsts[c]);
+  }
+  free(slot);
+  free(base);
+}
+
+template <int NJ>
+static void decode_team_d(int spec, const std::vector<uint8_t> &in, uint8_t *out, uint64_t N,
+                          uint32_t S, uint32_t U, uint32_t tb, uint64_t seed) {
+  switch (spec) {
+    case 1: decode_team_all<NJ, 1>(in, out, N, S, U, tb, seed); break;
+    case 2: decode_team_all<NJ, 2>(in, out, N, S, U, tb, seed); break;
+    case 4: decode_team_all<NJ, 4>(in, out, N, S, U, tb, seed); break;
+    default: fprintf(stderr, "spec bits\n"); exit(2);
+  }
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) return 2;
   const std::string mode = argv[1];
@@ -185,6 +255,22 @@ int main(int argc, char **argv) {
     write_file(argv[8], out);
     return 0;
   }
-  fprintf(stderr, "usage: wave_emu enc|dec ...\n");
+  if (mode == "tdec" && argc == 10) {
+    const std::vector<uint8_t> in = read_file(argv[2]);
+    const uint64_t N = strtoull(argv[3], nullptr, 10);
+    const uint32_t S = atoi(argv[4]), U = atoi(argv[5]), tb = atoi(argv[6]);
+    const int spec = atoi(argv[7]);
+    const uint64_t seed = strtoull(argv[8], nullptr, 10);
+    std::vector<uint8_t> out(N);
+    if (U <= 64)
+      decode_team_d<1>(spec, in, out.data(), N, S, U, tb, seed);
+    else if (U <= 128)
+      decode_team_d<2>(spec, in, out.data(), N, S, U, tb, seed);
+    else
+      decode_team_d<4>(spec, in, out.data(), N, S, U, tb, seed);
+    write_file(argv[9], out);
+    return 0;
+  }
+  fprintf(stderr, "usage: wave_emu enc|dec|tdec ...\n");
   return 2;
 }

@@ -6,9 +6,14 @@
 // wave_sync); when all 64 are there, the operation is resolved and every
 // lane continues.  Cross-lane LDS traffic is therefore ordered only at those
 // points, so device code that relies on lock step without a wave_sync shows
-// up as a mismatch here.  LDS and every global buffer are separate heap
-// blocks of their exact sizes, so AddressSanitizer reports any access out of
-// bounds (the GPU would silently drop an out-of-range LDS access).
+// up as a mismatch here.  A workgroup of several waves (run_team) steps its
+// waves in a seeded random order, one cross-lane operation at a time, and
+// holds each wave at a workgroup barrier (team_sync) until every unfinished
+// wave is there, so LDS traffic between waves that is not ordered by a
+// barrier or a flag varies from seed to seed.  LDS and every global buffer
+// are separate heap blocks of their exact sizes, so AddressSanitizer reports
+// any access out of bounds (the GPU would silently drop an out-of-range LDS
+// access).
 #ifndef CUCM_WAVE_EMULATION_H
 #define CUCM_WAVE_EMULATION_H
 #include <sanitizer/common_interface_defs.h>
@@ -65,7 +70,7 @@ emu_fiber_swap:
 
 constexpr int kLanes = 64;
 constexpr size_t kStack = 1 << 20;
-enum Kind { kNone, kRead, kFirst, kBallot, kSync };
+enum Kind { kNone, kRead, kFirst, kBallot, kSync, kBarrier };
 
 struct Wave {
   void *sched_sp = nullptr;
@@ -132,10 +137,9 @@ inline uint64_t meet(Kind k, uint64_t v, uint32_t arg, int line) {
   }
 }
 
-// Runs body(lane) for the 64 lanes of one wave to completion.
-inline void run_wave(const std::function<void(uint32_t)> &body) {
+// Creates the 64 lane fibers of a wave that will run body(lane).
+inline Wave *make_wave(const std::function<void(uint32_t)> &body) {
   Wave *w = new Wave();
-  current() = w;
   w->body = body;
   for (int l = 0; l < kLanes; ++l) {
     w->stack[l].assign(kStack, 0);
@@ -148,35 +152,91 @@ inline void run_wave(const std::function<void(uint32_t)> &body) {
     f[7] = nullptr;
     w->sp[l] = (void *)top;
   }
-  for (;;) {
-    for (int l = 0; l < kLanes; ++l)
-      if (!w->done[l]) to_lane(*w, l);
-    int ndone = 0;
-    for (int l = 0; l < kLanes; ++l) ndone += w->done[l];
-    if (ndone == kLanes) break;
-    if (ndone) {
-      int at = 0;
-      while (w->done[at]) ++at;
-      fprintf(stderr, "wave emulation: %d lanes finished while others wait at line %d\n", ndone,
-              w->line[at]);
+  return w;
+}
+
+// Runs every lane of the wave up to its next cross-lane operation.  Returns
+// the operation they all wait at (kNone once the wave has finished); wave
+// operations are resolved, a workgroup barrier is left pending.
+inline Kind step_wave(Wave *w) {
+  current() = w;
+  for (int l = 0; l < kLanes; ++l)
+    if (!w->done[l]) to_lane(*w, l);
+  current() = nullptr;
+  int ndone = 0;
+  for (int l = 0; l < kLanes; ++l) ndone += w->done[l];
+  if (ndone == kLanes) return kNone;
+  if (ndone) {
+    int at = 0;
+    while (w->done[at]) ++at;
+    fprintf(stderr, "wave emulation: %d lanes finished while others wait at line %d\n", ndone,
+            w->line[at]);
+    abort();
+  }
+  for (int l = 1; l < kLanes; ++l)
+    if (w->kind[l] != w->kind[0] || w->line[l] != w->line[0]) {
+      fprintf(stderr,
+              "wave emulation: divergent cross-lane operation: lane 0 at line %d, lane %d at "
+              "line %d\n",
+              w->line[0], l, w->line[l]);
       abort();
     }
-    for (int l = 1; l < kLanes; ++l)
-      if (w->kind[l] != w->kind[0] || w->line[l] != w->line[0]) {
-        fprintf(stderr,
-                "wave emulation: divergent cross-lane operation: lane 0 at line %d, lane %d at "
-                "line %d\n",
-                w->line[0], l, w->line[l]);
-        abort();
-      }
+  if (w->kind[0] != kBarrier) {
     w->ballot = 0;
     for (int l = 0; l < kLanes; ++l) {
       w->snap[l] = w->val[l];
       if (w->val[l]) w->ballot |= 1ull << l;
     }
   }
-  current() = nullptr;
+  return w->kind[0];
+}
+
+// Runs body(lane) for the 64 lanes of one wave to completion.
+inline void run_wave(const std::function<void(uint32_t)> &body) {
+  Wave *w = make_wave(body);
+  for (;;) {
+    const Kind k = step_wave(w);
+    if (k == kNone) break;
+    if (k == kBarrier) {
+      fprintf(stderr, "wave emulation: workgroup barrier in a one-wave run (line %d)\n", w->line[0]);
+      abort();
+    }
+  }
   delete w;
+}
+
+// Runs body(wave, lane) for a workgroup of `waves` waves to completion.  The
+// next wave to step is drawn from a generator seeded with `seed`; a wave at
+// a barrier waits until every unfinished wave has reached one.
+inline void run_team(uint32_t waves, uint64_t seed,
+                     const std::function<void(uint32_t, uint32_t)> &body) {
+  std::vector<Wave *> ws;
+  for (uint32_t i = 0; i < waves; ++i)
+    ws.push_back(make_wave([&body, i](uint32_t lane) { body(i, lane); }));
+  std::vector<int> state(waves, 0);  // 0 runnable, 1 at a barrier, 2 finished
+  uint64_t rng = seed * 0x9e3779b97f4a7c15ull + 1;
+  for (;;) {
+    std::vector<uint32_t> run;
+    for (uint32_t i = 0; i < waves; ++i)
+      if (state[i] == 0) run.push_back(i);
+    if (run.empty()) {
+      bool any = false;
+      for (uint32_t i = 0; i < waves; ++i)
+        if (state[i] == 1) {
+          state[i] = 0;
+          any = true;
+        }
+      if (!any) break;
+      continue;
+    }
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    const uint32_t i = run[rng % run.size()];
+    const Kind k = step_wave(ws[i]);
+    state[i] = k == kNone ? 2 : (k == kBarrier ? 1 : 0);
+  }
+  for (Wave *w : ws) delete w;
 }
 
 }  // namespace emu
@@ -195,12 +255,17 @@ inline uint64_t wave_ballot_at(bool p, int line) { return emu::meet(emu::kBallot
 #define wave_ballot(p) wave_ballot_at((p), __LINE__)
 inline void wave_sync_at(int line) { emu::meet(emu::kSync, 0, 0, line); }
 #define wave_sync() wave_sync_at(__LINE__)
+inline void team_sync_at(int line) { emu::meet(emu::kBarrier, 0, 0, line); }
+#define team_sync() team_sync_at(__LINE__)
+#define team_sync_lds() team_sync_at(__LINE__)
 #define vm_drain() wave_sync_at(__LINE__)
 inline uint32_t clz32(uint32_t x) { return x ? (uint32_t)__builtin_clz(x) : 32u; }
 inline uint64_t shader_clock() { return 0; }
 inline void wait_vm() {}
 inline void sched_fence() {}
 inline void wait_lgkm() {}
+inline void flag_publish(uint32_t *p, uint32_t v) { *p = v; }
+inline uint32_t flag_read(const uint32_t *p) { return *p; }
 inline uint32_t lds_cas(uint32_t *p, uint32_t cmp, uint32_t v) {
   const uint32_t o = *p;
   if (o == cmp) *p = v;
