@@ -129,7 +129,8 @@ class Context:
         check(lib().cucm_ctx_set_waves_per_block(self.handle, waves))
 
     def set_team(self, mode):
-        """Decoder team execution: 0 automatic, 1 never, 2 always."""
+        """Team execution (one chunk per workgroup of four waves): 0 automatic,
+        1 never, 2 always."""
         check(lib().cucm_ctx_set_team(self.handle, mode))
 
     PHASES = ["init", "stage", "hash+gather", "tables+mix", "coder", "mixer+collide", "resolve",

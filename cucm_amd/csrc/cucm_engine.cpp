@@ -208,8 +208,8 @@ struct Shape {
 };
 
 // Team workgroups the launch can hold at once (0 if a team does not fit).
-uint64_t team_capacity(const cucm_ctx *c, uint32_t Umax, uint32_t spec) {
-  const uint32_t lds = team_codec_lds_bytes(Umax, spec);
+uint64_t team_capacity(const cucm_ctx *c, uint32_t Umax, uint32_t spec, bool dec) {
+  const uint32_t lds = team_codec_lds_bytes(Umax, spec, dec);
   if (lds > c->lds_block) return 0;
   return (uint64_t)c->cus * std::max<uint32_t>(1, c->lds_cu / lds);
 }
@@ -219,7 +219,7 @@ uint64_t team_capacity(const cucm_ctx *c, uint32_t Umax, uint32_t spec) {
 // memory traffic, bounds the launch, and the team's extra waves shorten it.
 bool use_team(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, bool dec) {
   if (c->team_mode == 1 || !team_supported(dec, spec)) return false;
-  const uint64_t cap = team_capacity(c, Umax, spec);
+  const uint64_t cap = team_capacity(c, Umax, spec, dec);
   if (cap == 0) return false;
   return c->team_mode == 2 || ntasks <= cap;
 }
@@ -228,7 +228,7 @@ Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spe
                  uint64_t slot_bytes, uint64_t mem_left) {
   Shape s;
   if (use_team(c, ntasks, Umax, spec, dec)) {
-    uint64_t blocks = std::min<uint64_t>(ntasks, team_capacity(c, Umax, spec));
+    uint64_t blocks = std::min<uint64_t>(ntasks, team_capacity(c, Umax, spec, dec));
     if (c->max_waves) blocks = std::min<uint64_t>(blocks, std::max<uint32_t>(1, c->max_waves / kTeamWaves));
     if (slot_bytes) blocks = std::min<uint64_t>(blocks, mem_left / slot_bytes);
     s.team = true;

@@ -73,6 +73,29 @@ __global__ void __launch_bounds__(256) decode_team_kernel(CodecArgs a) {
   }
 }
 
+// The team encoder: one chunk per workgroup of kTeamWaves waves.
+template <int NJ>
+__global__ void __launch_bounds__(256) encode_team_kernel(CodecArgs a) {
+  constexpr uint32_t Umax = 64u * NJ;
+  extern __shared__ __align__(16) uint8_t lds[];
+  LDS DevTables *T = (LDS DevTables *)lds;
+  stage_tables(a.tabs, T);
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  LDS uint8_t *tbase = (LDS uint8_t *)lds + align16(sizeof(DevTables));
+  const TeamMem X = carve_team_enc(tbase, Umax, wid);
+  GLB uint8_t *slot = (GLB uint8_t *)a.pool + (uint64_t)blockIdx.x * a.slot_bytes;
+  for (;;) {
+    if (threadIdx.x == 0) X.ctl[0] = atomicAdd(a.counter, 1u);
+    team_sync();
+    const uint32_t t = __builtin_amdgcn_readfirstlane(X.ctl[0]);
+    team_sync();
+    if (t >= a.ntasks) break;
+    const Task tk = a.tasks[t];
+    encode_team<NJ>(a, tk, t, X, T, slot, Umax, wid, lane);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Container assembly: per job, per batch: LE u32 blob length, BE end
 // offsets E_c = 4C + sum_{j<=c} size_j - 1, payloads back to back
@@ -224,18 +247,30 @@ uint32_t codec_lds_bytes(uint32_t Umax, uint32_t spec_bits, bool decode, uint32_
   return align16(sizeof(DevTables)) + waves * wave_lds_bytes(unit_capacity(Umax), P, decode);
 }
 
-uint32_t team_codec_lds_bytes(uint32_t Umax, uint32_t spec_bits) {
-  return align16(sizeof(DevTables)) + team_lds_bytes(unit_capacity(Umax), spec_bits);
+uint32_t team_codec_lds_bytes(uint32_t Umax, uint32_t spec_bits, bool decode) {
+  return align16(sizeof(DevTables)) + (decode ? team_lds_bytes(unit_capacity(Umax), spec_bits)
+                                              : team_enc_lds_bytes(unit_capacity(Umax)));
 }
 
 bool team_supported(bool decode, uint32_t spec_bits) {
-  return decode && spec_bits >= 1 && spec_bits <= kTeamMaxSpec && (8 % spec_bits) == 0;
+  return !decode || (spec_bits >= 1 && spec_bits <= kTeamMaxSpec && (8 % spec_bits) == 0);
+}
+
+template <int NJ>
+static hipError_t launch_team_enc(const CodecArgs &a, uint32_t blocks, hipStream_t stream) {
+  const uint32_t lds = team_codec_lds_bytes(64u * NJ, 0, false);
+  hipError_t e = hipFuncSetAttribute((const void *)encode_team_kernel<NJ>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((encode_team_kernel<NJ>), dim3(blocks), dim3(64 * kTeamWaves), lds, stream,
+                     a);
+  return hipGetLastError();
 }
 
 template <int NJ, int D>
 static hipError_t launch_team_one(const CodecArgs &a, uint32_t Umax, uint32_t blocks,
                                   hipStream_t stream) {
-  const uint32_t lds = team_codec_lds_bytes(Umax, D);
+  const uint32_t lds = team_codec_lds_bytes(Umax, D, true);
   hipError_t e = hipFuncSetAttribute((const void *)decode_team_kernel<NJ, D>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -283,6 +318,11 @@ hipError_t launch_codec(bool decode, const CodecArgs &a, uint32_t Umax, uint32_t
   const int nj = Umax <= 64 ? 1 : (Umax <= 128 ? 2 : 4);
   if (team) {
     if (!team_supported(decode, a.spec_bits)) return hipErrorInvalidValue;
+    if (!decode) {
+      if (nj == 1) return launch_team_enc<1>(a, blocks, stream);
+      if (nj == 2) return launch_team_enc<2>(a, blocks, stream);
+      return launch_team_enc<4>(a, blocks, stream);
+    }
     if (nj == 1) return launch_team<1>(a, Umax, blocks, stream);
     if (nj == 2) return launch_team<2>(a, Umax, blocks, stream);
     return launch_team<4>(a, Umax, blocks, stream);

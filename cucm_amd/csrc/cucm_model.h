@@ -149,7 +149,7 @@ namespace cucm {
 uint32_t codec_lds_bytes(uint32_t Umax, uint32_t spec_bits, bool decode, uint32_t waves);
 // LDS of one team workgroup (one chunk per workgroup of several waves) and
 // whether a launch can run as teams.
-uint32_t team_codec_lds_bytes(uint32_t Umax, uint32_t spec_bits);
+uint32_t team_codec_lds_bytes(uint32_t Umax, uint32_t spec_bits, bool decode);
 bool team_supported(bool decode, uint32_t spec_bits);
 hipError_t launch_codec(bool decode, const CodecArgs &a, uint32_t Umax, uint32_t blocks,
                         uint32_t waves_per_block, bool team, hipStream_t stream);

@@ -47,6 +47,8 @@ struct TeamMem {
   LDS int16_t *sm;     // [8][256] stretch(statep >> 20) of the frozen statep
   LDS uint32_t *pcb;   // [NB][P] pr | cxt << 16 of every path of a round, per path buffer
   LDS uint32_t *cand;  // [2^D][P][8] next round's slots per outcome of the current round
+  LDS uint8_t *ubw;    // encoder: [Umax / 8] the walked unit's bytes
+  LDS uint16_t *t0;    // encoder: [256] running order-0 state by h0 (kNoState: unit start)
   LDS uint32_t *ctl;   // [16] 0: task index, 1: candidate round id, 2-7: two published
                        // (history lo, history hi, partial byte) states, by round parity
 };
@@ -100,6 +102,8 @@ CUCM_DEV TeamMem carve_team(LDS uint8_t *base, uint32_t Umax, uint32_t D, uint32
   x.sm = (LDS int16_t *)(base + l.sm);
   x.pcb = (LDS uint32_t *)(base + l.pcb);
   x.cand = (LDS uint32_t *)(base + l.cand);
+  x.ubw = nullptr;
+  x.t0 = nullptr;
   x.ctl = (LDS uint32_t *)(base + l.ctl);
   return x;
 }
@@ -150,6 +154,37 @@ CUCM_DEV void team_init_model(const CodecArgs &a, const Task &tk, const TeamMem 
   }
   for (uint32_t i = tid; i < 8 + Umax / 8 + 8; i += NT) M.ub[i] = 0;
   if (tid < 16) X.ctl[tid] = tid == 1 ? kEmpty : 0u;
+}
+
+// init_model for the encoder team (thread tid of kTeamWaves * 64).
+CUCM_DEV void team_init_enc(const CodecArgs &a, const Task &tk, const TeamMem &X,
+                            GLB uint8_t *slot, uint32_t Umax, uint32_t tid) {
+  constexpr uint32_t NT = kTeamWaves * 64;
+  const WaveMem &M = X.M;
+  const uint64_t n16 = (7ull << tk.tb) >> 4;
+  GLB u32x4 *dst = (GLB u32x4 *)slot;
+  if (tk.flags & kTaskInitModel) {
+    const DevInitModel *im = a.init + tk.model;
+    const GLB u32x4 *src = (const GLB u32x4 *)im->states;
+    for (uint64_t i = tid; i < n16; i += NT) dst[i] = src[i];
+    for (uint32_t i = tid; i < 8 * 256; i += NT) M.stp[i] = im->statep[i >> 8][i & 255];
+    for (uint32_t i = tid; i < kMixCtx * kOrders; i += NT) M.w[i] = im->w[i];
+    for (uint32_t i = tid; i < 256; i += NT) M.st0[i] = im->st0[i];
+  } else {
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    for (uint64_t i = tid; i < n16; i += NT) dst[i] = z;
+    for (uint32_t i = tid; i < 8 * 256; i += NT) M.stp[i] = 1u << 31;
+    for (uint32_t i = tid; i < kMixCtx * kOrders; i += NT) M.w[i] = 0;
+    for (uint32_t i = tid; i < 256; i += NT) M.st0[i] = 0;
+  }
+  const uint32_t TS = table_size(Umax);
+  for (uint32_t i = tid; i < 7 * TS; i += NT) {
+    M.tkey[i] = kEmpty;
+    M.tval[i] = kNoState;
+  }
+  for (uint32_t i = tid; i < Umax; i += NT) M.rprm[i] = 0;
+  for (uint32_t i = tid; i < 8 + Umax / 8 + 8; i += NT) M.ub[i] = 0;
+  for (uint32_t i = tid; i < 256; i += NT) X.t0[i] = kNoState;
 }
 
 // The leader's gathers of a round from slot sets wave 3 precomputed for
@@ -334,11 +369,11 @@ CUCM_DEV void team_resolve_byte(const TeamMem &X, const LDS DevTables *T, uint32
   }
 }
 
-// Wave 2: the statep walk of update_states' second part for byte b on stpn
-// (lane k < 8 walks order k).  The walk restarts at every byte: the entries
+// Wave 2: the statep walk of update_states' second part for byte b (its
+// bits y MSB first in ybyte) on stpn (lane k < 8 walks order k).  The walk restarts at every byte: the entries
 // of its first two bits are read after the previous byte's writes, so it
 // never looks into a byte that is not resolved yet.
-#define CUCM_TWALK_BYTE(X, T, b, Umax, lane)                                  \
+#define CUCM_TWALK_BYTE(X, T, b, ybyte, Umax, lane)                           \
   do {                                                                        \
     const uint32_t wk_ = min((lane), 7u);                                     \
     w_sp = (X).stpn + wk_ * 256;                                              \
@@ -357,7 +392,7 @@ CUCM_DEV void team_resolve_byte(const TeamMem &X, const LDS DevTables *T, uint32
     w_f1 = true;                                                              \
     w_f2 = false;                                                             \
     w_pnprev = w_dfprev = 0;                                                  \
-    const uint32_t wyb_ = (X).M.ub[8 + (b)];                                  \
+    const uint32_t wyb_ = (ybyte);                                            \
     CUCM_WALK_STEP(0, (wyb_ >> 7) & 1u);                                      \
     CUCM_WALK_STEP(1, (wyb_ >> 6) & 1u);                                      \
     CUCM_WALK_STEP(2, (wyb_ >> 5) & 1u);                                      \
@@ -552,12 +587,12 @@ CUCM_DEV void team_walker(const Task &tk, const TeamMem &X, const LDS DevTables 
   for (uint32_t base = 0; base < L; base += U / 8) {
     const uint32_t nblk = min(U / 8, L - base);
     for (uint32_t b = 0; b <= nblk; ++b) {
-      if (b >= 2) CUCM_TWALK_BYTE(X, T, b - 2, Umax, lane);
+      if (b >= 2) CUCM_TWALK_BYTE(X, T, b - 2, X.M.ub[8 + b - 2], Umax, lane);
       clk.mark(7);
       team_sync_lds();
       clk.mark(3);
     }
-    CUCM_TWALK_BYTE(X, T, nblk - 1, Umax, lane);
+    CUCM_TWALK_BYTE(X, T, nblk - 1, X.M.ub[8 + nblk - 1], Umax, lane);
     clk.mark(7);
     team_sync();
     team_refresh_sm(X, T, 2, lane);
@@ -622,6 +657,343 @@ CUCM_DEV void decode_team(const CodecArgs &a, const Task &tk, uint32_t t, const 
   // every wave counts the chunk's rounds
   const uint32_t U = tk.U;
   for (uint32_t base = 0; base < tk.L; base += U / 8) rid += min(U / 8, tk.L - base) * (8u / D);
+  clk.flush(a.prof, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Team encoder: one chunk per workgroup of kTeamWaves waves.  Per unit u:
+//   P1  waves 0, 1 stage, hash, gather and insert unit u (half the bits
+//       each); wave 2 walks unit u - 1's statep recurrences; wave 3 codes
+//       unit u - 1's bits;
+//   P2  waves 0, 1 predict unit u, keeping each bit's mixer deltas in
+//       registers;
+//   P3  waves 0, 1 add the deltas; wave 1 resolves the unit's states byte by
+//       byte; wave 2 keeps the unit's bytes for its walk;
+//   P4  all four write the touched slots back and clear the tables (a full
+//       barrier: the write-backs land before unit u + 1's gathers).
+// The model is updated in place: nothing reads statep during P1 but the
+// walk, and predictions (P2) come after it.
+// ---------------------------------------------------------------------------
+
+struct TeamEncLayout {
+  WaveLayout w;
+  uint32_t ubw, t0, dummy, ctl, total;
+};
+
+CUCM_HDI TeamEncLayout team_enc_layout(uint32_t Umax) {
+  TeamEncLayout t;
+  t.w = wave_layout(Umax, 0, false);
+  uint32_t off = t.w.total;
+  t.ubw = off;
+  off = align16(off + Umax / 8);
+  t.t0 = off;
+  off = align16(off + 256 * 2);
+  t.dummy = off;
+  off = align16(off + kTeamWaves * 64 * 4);
+  t.ctl = off;
+  off = align16(off + 16 * 4);
+  t.total = off;
+  return t;
+}
+
+CUCM_HD uint32_t team_enc_lds_bytes(uint32_t Umax) { return team_enc_layout(Umax).total; }
+
+// The encoder team's views: the walk runs on the model's own statep and the
+// resolution on its own order-0 states.
+CUCM_DEV TeamMem carve_team_enc(LDS uint8_t *base, uint32_t Umax, uint32_t wid) {
+  const TeamEncLayout l = team_enc_layout(Umax);
+  TeamMem x;
+  x.M = carve(base, Umax, 0, false);
+  x.M.dummy = (LDS uint32_t *)(base + l.dummy) + 64 * wid;
+  x.stpn = x.M.stp;
+  x.wd = nullptr;
+  x.st0r = x.M.st0;
+  x.sm = nullptr;
+  x.pcb = nullptr;
+  x.cand = nullptr;
+  x.ubw = base + l.ubw;
+  x.t0 = (LDS uint16_t *)(base + l.t0);
+  x.ctl = (LDS uint32_t *)(base + l.ctl);
+  return x;
+}
+
+// encode_predict for the bits r = r0 + lane + 64j of one issuing wave, with
+// each bit's mixer deltas (cucm_compress_v2.cu:794-818: err x stretch of the
+// unit-start statep) and context returned in registers.
+template <int NJ>
+CUCM_DEV void encode_predict_team(const WaveMem &M, const LDS DevTables *T, uint32_t nvalid,
+                                  uint32_t Umax, uint32_t lane, uint32_t r0,
+                                  const uint32_t sl[NJ][8], const uint32_t g[NJ][8],
+                                  const uint32_t idx[NJ][8], int32_t dl[NJ][8],
+                                  uint32_t cx[NJ]) {
+  int32_t smp[NJ][8];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) smp[j][k] = T->stretch[M.stp[k * 256 + g[j][k]] >> 20];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const uint32_t r = r0 + lane + 64u * j;
+    const bool ok = r < nvalid;
+    const uint32_t rr = min(r, Umax - 1);
+    uint32_t active = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) active += (smp[j][k] != 1);
+    const uint32_t cxt = active + (uint32_t)(M.ub[8 + (rr >> 3) - 1] >> 5) * 10u;
+    const LDS u32x4 *wr = (const LDS u32x4 *)(M.w + cxt * 8u);
+    const u32x4 wa = wr[0], wb = wr[1];
+    const int32_t w[8] = {(int32_t)wa.x, (int32_t)wa.y, (int32_t)wa.z, (int32_t)wa.w,
+                          (int32_t)wb.x, (int32_t)wb.y, (int32_t)wb.z, (int32_t)wb.w};
+    const uint32_t pr = mix_weights(w, smp[j], T->squash);
+    const uint32_t y = ((uint32_t)M.ub[8 + (rr >> 3)] >> (7 - (rr & 7u))) & 1u;
+    const int32_t err = ((int32_t)(y << 12) - (int32_t)pr) * 7;
+    cx[j] = cxt;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dl[j][k] = ok ? (int32_t)(((int64_t)err * smp[j][k]) >> 16) : 0;
+    if (ok) {
+      M.rprm[rr] = (uint16_t)((pr << 1) | y);
+      M.ridx[rr] = (uint16_t)sl[j][0];
+      M.rg[rr] = (uint8_t)g[j][0];
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        M.ridx[k * Umax + rr] = (uint16_t)idx[j][k];
+        M.rg[k * Umax + rr] = (uint8_t)g[j][k];
+      }
+    }
+  }
+  wave_sync();
+}
+
+// P3 of wave 1: the state each StateMap update of the unit sees (update_states'
+// resolution for nbytes bytes; lane (k, j) = (order, bit of the byte)).  The bytes with an in-byte collision
+// are found first; each byte's slot, unit-start state and bit are loaded a
+// byte ahead, so a byte costs the dependent state lookup, nex lookup and
+// store alone.
+CUCM_DEV void team_resolve_unit(const TeamMem &X, const LDS DevTables *T, uint32_t nbytes,
+                                uint32_t Umax, uint32_t lane) {
+  const WaveMem &M = X.M;
+  const uint32_t TS = table_size(Umax);
+  // bytes where two bits of an order >= 1 hash to one slot
+  uint32_t cmask = 0;
+  for (uint32_t base = 0; base < 7 * nbytes; base += 64) {
+    const uint32_t it = base + lane;
+    bool coll = false;
+    if (it < 7 * nbytes) {
+      const uint32_t b = it / 7, k = 1 + it % 7;
+      const u32x4 v = *(const LDS u32x4 *)(M.ridx + k * Umax + 8 * b);
+      const uint32_t e[8] = {v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16,
+                             v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16};
+#pragma unroll
+      for (int x = 0; x < 8; ++x)
+#pragma unroll
+        for (int z = x + 1; z < 8; ++z) coll |= (e[x] == e[z]);
+    }
+    uint64_t bal = wave_ballot(coll);
+    while (bal) {
+      const uint32_t bit = (uint32_t)__builtin_ctzll(bal);
+      bal &= bal - 1;
+      cmask |= 1u << ((base + bit) / 7);
+    }
+  }
+  const uint32_t k = lane >> 3, j = lane & 7u;
+  // running state per slot: order 0 in t0 (by h0), orders 1..7 in the
+  // distinct-slot tables; both read kNoState until the unit touches a slot
+  LDS uint16_t *tv = k ? M.tval + (k - 1) * TS : X.t0;
+  LDS uint8_t *rgk = M.rg + k * Umax;
+  const LDS uint16_t *rik = M.ridx + k * Umax;
+  uint32_t i = rik[j], g = rgk[j], y = ((uint32_t)M.ub[8] >> (7 - j)) & 1u;
+  for (uint32_t b = 0; b < nbytes; ++b) {
+    const uint32_t r = 8 * b + j;
+    const uint32_t bn = min(b + 1, nbytes - 1);
+    const uint32_t i_n = rik[8 * bn + j], g_n = rgk[8 * bn + j];
+    const uint32_t y_n = ((uint32_t)M.ub[8 + bn] >> (7 - j)) & 1u;
+    const uint32_t reps = ((cmask >> b) & 1u) ? 8u : 1u;
+    for (uint32_t q = 0; q < reps; ++q) {
+      if (reps == 1u || q == j) {
+        const uint32_t v = tv[i];
+        const uint32_t st = v > 255u ? g : v;
+        tv[i] = (uint16_t)T->nex[2 * st + y];
+        rgk[r] = (uint8_t)st;
+      }
+      lane_step();
+    }
+    i = i_n;
+    g = g_n;
+    y = y_n;
+  }
+  wave_sync();
+}
+
+// P4 part of wave `part` (0..3): orders 1 + part and 5 + part written back
+// and their tables cleared (wave 3 also folds order 0's states into st0).
+CUCM_DEV void team_enc_writeback(const TeamMem &X, GLB uint8_t *slot, uint32_t tb,
+                                 uint32_t nvalid, uint32_t Umax, uint32_t part, uint32_t lane) {
+  const WaveMem &M = X.M;
+  const uint32_t TS = table_size(Umax);
+  if (part == 3) {
+    // order 0: the final state of every touched h0 back into the model
+    for (uint32_t r = lane; r < nvalid; r += 64) {
+      const uint32_t h0 = M.ridx[r];
+      M.st0[h0] = (uint8_t)X.t0[h0];
+    }
+    wave_sync();
+    for (uint32_t r = lane; r < nvalid; r += 64) X.t0[M.ridx[r]] = kNoState;
+  }
+#pragma unroll 1
+  for (uint32_t k = 1 + part; k < 8; k += 4) {
+    GLB uint8_t *tab = slot + ((uint64_t)(k - 1) << tb);
+    for (uint32_t r = lane; r < nvalid; r += 64) {
+      const uint32_t i = M.ridx[k * Umax + r];
+      tab[M.tkey[(k - 1) * TS + i]] = (uint8_t)M.tval[(k - 1) * TS + i];
+    }
+    wave_sync();
+    for (uint32_t r = lane; r < nvalid; r += 64) {
+      const uint32_t i = M.ridx[k * Umax + r];
+      M.tkey[(k - 1) * TS + i] = kEmpty;
+      M.tval[(k - 1) * TS + i] = kNoState;
+    }
+  }
+}
+
+// Unit schedule shared by the roles: units u = 0..NU-1 cover the coded bits
+// (the tail-padded ones included); nvalid(u) bits of unit u carry data.
+struct EncUnits {
+  uint64_t vbits, coded_end;
+  uint32_t U, NU;
+  CUCM_DEV explicit EncUnits(const Task &tk) {
+    U = tk.U;
+    vbits = 8ull * tk.L;
+    coded_end = (tk.flags & kTaskTailPad) ? 8ull * tk.S : vbits;
+    NU = (uint32_t)((coded_end + U - 1) / U);
+  }
+  CUCM_DEV uint32_t nvalid(uint32_t u) const {
+    const uint64_t base = (uint64_t)u * U;
+    return (u < NU && base < vbits) ? (uint32_t)min((uint64_t)U, vbits - base) : 0u;
+  }
+  CUCM_DEV uint32_t ncoded(uint32_t u) const {
+    return (uint32_t)min((uint64_t)U, coded_end - (uint64_t)u * U);
+  }
+};
+
+// Waves 0 and 1: issue, predict, deltas; wave 1 also resolves.
+template <int NJ>
+CUCM_DEV void team_enc_issuer(const Task &tk, const TeamMem &X, const LDS DevTables *T,
+                              GLB uint8_t *slot, uint32_t Umax, uint32_t w, uint32_t lane,
+                              PhaseClock &clk) {
+  constexpr int NJW = NJ >= 2 ? NJ / 2 : 1;
+  const uint32_t r0 = w * 64u * NJW;
+  const WaveMem &M = X.M;
+  const EncUnits E(tk);
+  const uint32_t tb = tk.tb, mask = (1u << tb) - 1u;
+  const GLB uint8_t *src = (const GLB uint8_t *)(uintptr_t)tk.src;
+  for (uint32_t u = 0; u <= E.NU; ++u) {
+    const uint32_t nvalid = E.nvalid(u);
+    uint32_t sl[NJW][8], g[NJW][8], idx[NJW][8];
+    if (nvalid)
+      encode_issue<NJW>(M, src, tk.L, (uint64_t)u * E.U, E.U, nvalid, slot, tb, mask, Umax, lane,
+                        sl, g, idx, r0);
+    clk.mark(2);
+    team_sync_lds();  // P1
+    if (!nvalid) continue;
+    int32_t dl[NJW][8];
+    uint32_t cx[NJW];
+    encode_predict_team<NJW>(M, T, nvalid, Umax, lane, r0, sl, g, idx, dl, cx);
+    clk.mark(3);
+    team_sync_lds();  // P2
+#pragma unroll
+    for (int j = 0; j < NJW; ++j)
+      if (r0 + lane + 64u * j < nvalid)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) lds_add(&M.w[cx[j] * 8 + k], dl[j][k]);
+    clk.mark(5);
+    if (w == 1) team_resolve_unit(X, T, nvalid / 8, Umax, lane);
+    clk.mark(6);
+    team_sync_lds();  // P3
+    team_enc_writeback(X, slot, tb, nvalid, Umax, w, lane);
+    team_sync();  // P4
+    clk.mark(8);
+  }
+}
+
+// Wave 2: walks unit u - 1 during P1 of unit u.
+CUCM_DEV void team_enc_walker(const Task &tk, const TeamMem &X, const LDS DevTables *T,
+                              GLB uint8_t *slot, uint32_t Umax, uint32_t lane, PhaseClock &clk) {
+  const WaveMem &M = X.M;
+  const EncUnits E(tk);
+  CUCM_WALK_STATE;
+  for (uint32_t u = 0; u <= E.NU; ++u) {
+    const uint32_t prev = u ? E.nvalid(u - 1) : 0u;
+    for (uint32_t b = 0; b < prev / 8; ++b) CUCM_TWALK_BYTE(X, T, b, X.ubw[b], Umax, lane);
+    clk.mark(7);
+    team_sync_lds();  // P1
+    const uint32_t nvalid = E.nvalid(u);
+    if (!nvalid) continue;
+    team_sync_lds();  // P2
+    for (uint32_t i = lane; i < nvalid / 8; i += 64) X.ubw[i] = M.ub[8 + i];
+    team_sync_lds();  // P3
+    team_enc_writeback(X, slot, tk.tb, nvalid, Umax, 2, lane);
+    team_sync();  // P4
+  }
+}
+
+// Wave 3: codes unit u - 1 during P1 of unit u (its (pr, y) read into
+// registers before unit u's predictions overwrite them), then flushes.
+template <int NJ>
+CUCM_DEV void team_enc_coder(const CodecArgs &a, const Task &tk, uint32_t t, const TeamMem &X,
+                             GLB uint8_t *slot, uint32_t Umax, uint32_t lane, PhaseClock &clk) {
+  const WaveMem &M = X.M;
+  const EncUnits E(tk);
+  GLB uint8_t *dst = (GLB uint8_t *)(uintptr_t)tk.dst;
+  const uint64_t cap = tk.aux;
+  uint32_t x1 = 0, x2 = 0xffffffffu, acc = 0;
+  uint32_t opos = 0, ovf = 0;
+  for (uint32_t u = 0; u <= E.NU; ++u) {
+    if (u) {
+      uint32_t pv[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) pv[j] = M.rprm[lane + 64u * j];
+      const uint32_t nc = E.ncoded(u - 1);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const uint32_t lo = 64u * j, hi = min(nc, lo + 64u);
+        for (uint32_t r = lo; r < hi; ++r) CUCM_CODE_BIT(lane_read(pv[j], r - lo));
+      }
+    }
+    clk.mark(4);
+    team_sync_lds();  // P1
+    const uint32_t nvalid = E.nvalid(u);
+    if (!nvalid) continue;
+    team_sync_lds();  // P2
+    team_sync_lds();  // P3
+    team_enc_writeback(X, slot, tk.tb, nvalid, Umax, 3, lane);
+    team_sync();  // P4
+  }
+  acc |= (x1 >> 24) << (8 * (opos & 3u));
+  ++opos;
+  const uint64_t wstart = (uint64_t)((opos - 1) & ~3u);
+  if (wstart + 4 <= cap) {
+    if (lane == 0) *(GLB uint32_t *)(dst + wstart) = acc;
+  } else {
+    ovf = 1u;
+  }
+  if (lane == 0) a.result[t] = (uint64_t)opos | (ovf ? kResultOverflow : 0ull);
+}
+
+// One chunk encoded by the team (all kTeamWaves waves call this together).
+template <int NJ>
+CUCM_DEV void encode_team(const CodecArgs &a, const Task &tk, uint32_t t, const TeamMem &X,
+                          const LDS DevTables *T, GLB uint8_t *slot, uint32_t Umax, uint32_t wid,
+                          uint32_t lane) {
+  PhaseClock clk(a.prof != nullptr);
+  team_init_enc(a, tk, X, slot, Umax, wid * 64 + lane);
+  team_sync();
+  clk.mark(0);
+  if (wid <= 1)
+    team_enc_issuer<NJ>(tk, X, T, slot, Umax, wid, lane, clk);
+  else if (wid == 2)
+    team_enc_walker(tk, X, T, slot, Umax, lane, clk);
+  else
+    team_enc_coder<NJ>(a, tk, t, X, slot, Umax, lane, clk);
   clk.flush(a.prof, lane);
 }
 

@@ -59,6 +59,11 @@ CUCM_DEV void wave_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// Marks a step of a chain whose cross-lane LDS order the hardware already
+// gives (a wave's LDS operations execute in program order, and a lane's own
+// store-then-load of a possibly equal address keeps its order): no wait is
+// emitted; the CPU wave emulation keeps its lanes in step here.
+CUCM_DEV void lane_step() { __builtin_amdgcn_wave_barrier(); }
 // Waits for this wave's outstanding vector memory operations: a wave's state
 // write-backs must land before its own next gathers.  No cache maintenance
 // is needed for a wave's own accesses.
@@ -549,14 +554,15 @@ CUCM_DEV void statep_walk(const WaveMem &M, const LDS DevTables *T, uint32_t Uma
 // Stages unit bytes, hashes every bit's 8 slots, issues the 7 state gathers
 // per bit and places every slot in the unit's distinct-slot tables (the
 // inserts run while the gathers are in flight).  Lane handles unit bits
-// r = lane + 64j; bits past nvalid compute in-bounds garbage and insert
+// r = r0 + lane + 64j; bits past nvalid compute in-bounds garbage and insert
 // nothing.
 template <int NJ>
 CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
                                              uint64_t base, uint32_t U, uint32_t nvalid,
                                              const GLB uint8_t *slot, uint32_t tb, uint32_t mask,
                                              uint32_t Umax, uint32_t lane, uint32_t sl[NJ][8],
-                                             uint32_t g[NJ][8], uint32_t idx[NJ][8]) {
+                                             uint32_t g[NJ][8], uint32_t idx[NJ][8],
+                                             uint32_t r0 = 0) {
   const uint32_t TS = table_size(Umax);
   // 8 history bytes and the unit's bytes (zero outside the chunk)
   const int64_t b0 = (int64_t)(base >> 3) - 8;
@@ -567,7 +573,7 @@ CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
   wave_sync();
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const uint32_t r = min(lane + 64u * j, Umax - 1);
+    const uint32_t r = min(r0 + lane + 64u * j, Umax - 1);
     const uint32_t bi = r >> 3, jb = r & 7u;
     const uint32_t cur = M.ub[8 + bi];
     const uint32_t h0 = (1u << jb) | (jb ? (cur >> (8 - jb)) : 0u);
@@ -588,7 +594,7 @@ CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
   uint32_t old[NJ][8];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const bool ok = lane + 64u * j < nvalid;
+    const bool ok = r0 + lane + 64u * j < nvalid;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
       idx[j][k] = sl[j][k] & (TS - 1);
@@ -601,7 +607,7 @@ CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
   for (int j = 0; j < NJ; ++j)
 #pragma unroll
     for (int k = 1; k < 8; ++k)
-      if ((lane + 64u * j < nvalid) && old[j][k] != kEmpty && old[j][k] != sl[j][k])
+      if ((r0 + lane + 64u * j < nvalid) && old[j][k] != kEmpty && old[j][k] != sl[j][k])
         pend |= 1u << (j * 8 + k);
   while (wave_ballot(pend != 0)) {
 #pragma unroll

@@ -199,12 +199,14 @@ int cucm_ctx_set_max_waves(cucm_ctx *ctx, uint32_t waves);
  * one LDS copy of the lookup tables; fewer per workgroup spreads a small
  * launch over more CUs.  Never changes the output. */
 int cucm_ctx_set_waves_per_block(cucm_ctx *ctx, uint32_t waves);
-/* Team execution of the decoder: one chunk per workgroup of 4 waves (the
- * leader runs the speculative round chain; the others record rounds, resolve
- * states, walk the StateMap recurrences and hash the next round's paths).
- * mode 0 (default): teams when every chunk of a launch can be resident as a
- * team at once and spec_bits is 1, 2 or 4; 1: never; 2: whenever spec_bits
- * allows.  Never changes the output. */
+/* Team execution: one chunk per workgroup of 4 waves.  Encoder: two waves
+ * hash, gather and predict a unit while one walks the previous unit's
+ * StateMap recurrences and one codes its bits.  Decoder: the leader runs the
+ * speculative round chain; the others record rounds, resolve states, walk
+ * the StateMap recurrences and hash the next round's paths.  mode 0
+ * (default): teams when every chunk of a launch can be resident as a team at
+ * once (decoder: spec_bits 1, 2 or 4); 1: never; 2: whenever allowed.  Never
+ * changes the output. */
 int cucm_ctx_set_team(cucm_ctx *ctx, uint32_t mode);
 
 /* ---- measurement probes (device 0 / current device) ------------------- */

@@ -1,8 +1,10 @@
-"""The team decoder (one chunk per workgroup of four waves, cucm_team.h) on
-the GPU: lossless on every configuration and speculation width it supports,
-with a prelearned model, with more chunks than resident teams (workgroups
-reuse their LDS and model slot from chunk to chunk), over many containers in
-one launch, and chosen automatically for a launch whose chunks all fit."""
+"""The team encoder and decoder (one chunk per workgroup of four waves,
+cucm_team.h) on the GPU: containers byte-identical to the oracle's and
+round trips lossless on every configuration and speculation width they
+support, with a prelearned model, with more chunks than resident teams
+(workgroups reuse their LDS and model slot from chunk to chunk), over many
+containers in one launch, and chosen automatically for a launch whose chunks
+all fit."""
 import numpy as np
 import pytest
 
@@ -46,11 +48,12 @@ CASES = [
 
 
 @pytest.mark.parametrize("kind,N,S,U,batch,P,tb,spec", CASES)
-def test_team_decoder_lossless(tctx, oracle, kind, N, S, U, batch, P, tb, spec):
+def test_team_codec_identical_and_lossless(tctx, oracle, kind, N, S, U, batch, P, tb, spec):
     data = _data(kind, N, 11)
     blob = oracle.compress(data, S, U, batch, P, tb)
     tctx.set_team(2)
     try:
+        assert tctx.compress(data, make_params(S, U, batch, P, tb)) == blob
         assert tctx.decompress(blob, spec) == data
     finally:
         tctx.set_team(0)
@@ -62,6 +65,8 @@ def test_team_workgroups_cycle_through_many_chunks(tctx, oracle):
     tctx.set_team(2)
     tctx.set_max_waves(8)  # two teams for 74 chunks
     try:
+        assert tctx.compress(data, make_params(4096, 128, 100, 0, 16)) == blob
+        assert tctx.stats()["waves"] == 8
         assert tctx.decompress(blob, 4) == data
         assert tctx.stats()["waves"] == 8
     finally:
@@ -74,6 +79,7 @@ def test_team_many_containers_one_launch(tctx, oracle):
     blobs = [oracle.compress(d, 4096, 64, 2, 0, 14) for d in datas]
     tctx.set_team(2)
     try:
+        assert tctx.compress_many(datas, make_params(4096, 64, 2, 0, 14)) == blobs
         assert tctx.decompress_many(blobs, 2) == datas
     finally:
         tctx.set_team(0)
@@ -83,7 +89,14 @@ def test_auto_mode_runs_few_chunks_as_teams(tctx, oracle):
     data = _data("structured", 40000, 9)
     p = make_params(8192, 128, 100, 0, 16)
     blob = tctx.compress(data, p)
+    assert tctx.stats()["waves"] == 20  # 5 chunks as teams
     assert blob == oracle.compress(data, 8192, 128, 100, 0, 16)
+    tctx.set_team(1)
+    try:
+        assert tctx.compress(data, p) == blob
+        assert tctx.stats()["waves"] == 5
+    finally:
+        tctx.set_team(0)
     for mode, spec in ((0, 4), (1, 4), (0, 8)):
         tctx.set_team(mode)
         try:

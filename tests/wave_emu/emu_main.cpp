@@ -6,6 +6,9 @@
 //       output: per chunk, a u32 LE payload length and the payload
 //   wave_emu dec <payloads> <N> <S> <U> <tb> <spec_bits> <output>
 //       payloads in the format enc writes; output: the N decoded bytes
+//   wave_emu tenc <input> <S> <U> <tb> <tail_pad> <seed> <output>
+//       enc through the team encoder (cucm_team.h): every chunk in turn on
+//       one workgroup of kTeamWaves waves and one LDS
 //   wave_emu tdec <payloads> <N> <S> <U> <tb> <spec_bits> <seed> <output>
 //       the same through the team decoder (cucm_team.h): every chunk in turn
 //       on one workgroup of kTeamWaves waves and one LDS, waves stepped in
@@ -134,6 +137,65 @@ static void decode_d(int spec, const std::vector<uint8_t> &pay, uint8_t *out, ui
   }
 }
 
+// All chunks through one encoder team, as the persistent team kernel runs them.
+template <int NJ>
+static std::vector<uint8_t> encode_team_all(const std::vector<uint8_t> &in, uint32_t S,
+                                            uint32_t U, uint32_t tb, bool pad, uint64_t seed) {
+  const uint64_t N = in.size();
+  const uint64_t C = (N + S - 1) / S;
+  const uint32_t Umax = 64u * NJ;
+  std::vector<Task> tasks(C);
+  std::vector<uint8_t *> srcs(C), dsts(C);
+  const uint32_t cap = align16(S + S / 2 + 256);
+  for (uint64_t c = 0; c < C; ++c) {
+    const uint32_t L = (uint32_t)std::min<uint64_t>(S, N - c * S);
+    srcs[c] = (uint8_t *)malloc(L ? L : 1);
+    memcpy(srcs[c], in.data() + c * S, L);
+    dsts[c] = (uint8_t *)malloc(cap);
+    Task &tk = tasks[c];
+    tk = Task{};
+    tk.src = (uint64_t)(uintptr_t)srcs[c];
+    tk.dst = (uint64_t)(uintptr_t)dsts[c];
+    tk.L = L;
+    tk.S = S;
+    tk.aux = cap;
+    tk.U = (uint16_t)U;
+    tk.tb = (uint8_t)tb;
+    tk.flags = (pad && c == C - 1) ? kTaskTailPad : 0;
+  }
+  const size_t tab = align16(sizeof(DevTables));
+  const size_t bytes = tab + team_enc_lds_bytes(Umax);
+  uint8_t *base = (uint8_t *)aligned_alloc(16, (bytes + 15) & ~(size_t)15);
+  memset(base, 0xa5, bytes);
+  memcpy(base, &host_tables(), sizeof(DevTables));
+  const DevTables *T = (const DevTables *)base;
+  uint8_t *slot = (uint8_t *)malloc((size_t)7 << tb);
+  std::vector<uint64_t> result(C, 0);
+  CodecArgs a{};
+  a.tasks = tasks.data();
+  a.ntasks = (uint32_t)C;
+  a.result = result.data();
+  emu::run_team(kTeamWaves, seed, [&](uint32_t wid, uint32_t lane) {
+    const TeamMem X = carve_team_enc(base + tab, Umax, wid);
+    for (uint64_t c = 0; c < C; ++c) encode_team<NJ>(a, tasks[c], (uint32_t)c, X, T, slot, Umax, wid, lane);
+  });
+  std::vector<uint8_t> out;
+  for (uint64_t c = 0; c < C; ++c) {
+    if (result[c] & kResultOverflow) {
+      fprintf(stderr, "staging overflow\n");
+      exit(3);
+    }
+    const uint32_t n = (uint32_t)result[c];
+    for (int i = 0; i < 4; ++i) out.push_back((uint8_t)(n >> (8 * i)));
+    out.insert(out.end(), dsts[c], dsts[c] + n);
+    free(srcs[c]);
+    free(dsts[c]);
+  }
+  free(slot);
+  free(base);
+  return out;
+}
+
 // All chunks through one team, as the persistent team kernel runs them.
 template <int NJ, int D>
 static void decode_team_all(const std::vector<uint8_t> &in, uint8_t *out, uint64_t N, uint32_t S,
@@ -255,6 +317,21 @@ int main(int argc, char **argv) {
     write_file(argv[8], out);
     return 0;
   }
+  if (mode == "tenc" && argc == 9) {
+    const std::vector<uint8_t> in = read_file(argv[2]);
+    const uint32_t S = atoi(argv[3]), U = atoi(argv[4]), tb = atoi(argv[5]);
+    const bool pad = atoi(argv[6]) != 0;
+    const uint64_t seed = strtoull(argv[7], nullptr, 10);
+    std::vector<uint8_t> out;
+    if (U <= 64)
+      out = encode_team_all<1>(in, S, U, tb, pad, seed);
+    else if (U <= 128)
+      out = encode_team_all<2>(in, S, U, tb, pad, seed);
+    else
+      out = encode_team_all<4>(in, S, U, tb, pad, seed);
+    write_file(argv[8], out);
+    return 0;
+  }
   if (mode == "tdec" && argc == 10) {
     const std::vector<uint8_t> in = read_file(argv[2]);
     const uint64_t N = strtoull(argv[3], nullptr, 10);
@@ -271,6 +348,6 @@ int main(int argc, char **argv) {
     write_file(argv[9], out);
     return 0;
   }
-  fprintf(stderr, "usage: wave_emu enc|dec|tdec ...\n");
+  fprintf(stderr, "usage: wave_emu enc|dec|tenc|tdec ...\n");
   return 2;
 }

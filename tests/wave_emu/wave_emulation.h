@@ -259,6 +259,7 @@ inline void team_sync_at(int line) { emu::meet(emu::kBarrier, 0, 0, line); }
 #define team_sync() team_sync_at(__LINE__)
 #define team_sync_lds() team_sync_at(__LINE__)
 #define vm_drain() wave_sync_at(__LINE__)
+#define lane_step() wave_sync_at(__LINE__)
 inline uint32_t clz32(uint32_t x) { return x ? (uint32_t)__builtin_clz(x) : 32u; }
 inline uint64_t shader_clock() { return 0; }
 inline void wait_vm() {}
