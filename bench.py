@@ -127,17 +127,22 @@ def stream_workload(args):
         "lossless": lossless,
         "container_bytes": clen,
         "waves": sc["waves"],
+        "compress_teams": sc["teams"],
+        "decompress_teams": sd["teams"],
         "model_bytes": sc["model_bytes"],
     }
     ms_per_step = 1000.0 * t_all / args.steps
     value = n / 1e6 / (t_all / args.steps)
-    # dominant kernel: the slower of the two launches
+    # dominant kernel: the slower of the two launches (a team kernel when the
+    # launch ran as teams of four waves, else the one-wave-per-chunk kernel)
     dom = "decode" if res["decompress_kernel_ms"] >= res["compress_kernel_ms"] else "encode"
+    teams = res["decompress_teams"] if dom == "decode" else res["compress_teams"]
+    kname = ("%s_team_kernel" % dom) if teams else ("codec_kernel<%s>" % dom)
     kms = max(res["compress_kernel_ms"], res["decompress_kernel_ms"])
     achieved = ALG_BYTES_PER_INPUT_BYTE * n / (kms / 1000.0) / 1e9
-    roofline = {"bound": "hbm", "kernel": "codec_kernel<%s>" % dom, "achieved": achieved,
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(dom, n)}
+                "traffic": measured_traffic(kname, n)}
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -163,7 +168,9 @@ def measured_traffic(kernel, n):
     path = os.path.join(ROOT, "profiles", "codec_kernel_traffic.json")
     if not os.path.exists(path):
         return None
-    t = json.load(open(path))[kernel]
+    t = json.load(open(path)).get(kernel)
+    if t is None:
+        return None
     return (t["fetch_bytes_per_input_byte"] + t["write_bytes_per_input_byte"]) * n
 
 

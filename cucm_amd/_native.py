@@ -64,6 +64,7 @@ class Stats(ctypes.Structure):
         ("in_bytes", ctypes.c_uint64),
         ("out_bytes", ctypes.c_uint64),
         ("model_bytes", ctypes.c_uint64),
+        ("teams", ctypes.c_uint64),
     ]
 
     def as_dict(self):

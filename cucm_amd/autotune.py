@@ -10,8 +10,10 @@ events) and picks:
     ratio on the sample is at least (1 - ratio_tol) x the ratio of the
     reference setting (2^18 slots, 256 KiB chunks);
   - launch settings that never change the container: the decoder's
-    speculation width (bits decided per round) and the waves per workgroup
-    (how many waves share one LDS copy of the lookup tables).
+    speculation width (bits decided per round), the waves per workgroup
+    (how many waves share one LDS copy of the lookup tables) and team
+    execution (one chunk per workgroup of four waves instead of one wave per
+    chunk).
 
 The score is sample bytes per second of compress + decompress kernel time.
 `choose` is the pure selection rule; `autotune_batch` runs the measurements.
@@ -29,6 +31,7 @@ REFERENCE = {"chunk_max": 256 * KiB, "table_bits": 18}
 STREAM_GRID = [(c, t) for c in (128 * KiB, 256 * KiB, 1024 * KiB) for t in (16, 18)]
 SPEC_GRID = (2, 4, 8)
 WPB_GRID = (0, 2, 4, 8)
+TEAM_SPEC_GRID = (2, 4)  # speculation widths the team decoder runs
 
 
 def stratified_sample(sizes, budget, seed=0):
@@ -70,7 +73,7 @@ def file_params(n, chunk_max, table_bits, unit_bits=128, batch_chunks=100):
     return make_params(int(max(1, min(n, chunk_max))), unit_bits, batch_chunks, 0, table_bits)
 
 
-def _measure(ctx, d_in, ins, lens, d_c, d_o, chunk_max, table_bits, spec, wpb, coffs):
+def _measure(ctx, d_in, ins, lens, d_c, d_o, chunk_max, table_bits, spec, wpb, coffs, team=False):
     params = [file_params(n, chunk_max, table_bits) for n in lens]
     caps = [output_cap(n, p) for n, p in zip(lens, params)]
     if any(c > int(coffs[i + 1] - coffs[i]) for i, c in enumerate(caps)):
@@ -78,6 +81,7 @@ def _measure(ctx, d_in, ins, lens, d_c, d_o, chunk_max, table_bits, spec, wpb, c
     cs = [d_c.addr + int(o) for o in coffs[:-1]]
     outs = [d_o.addr + (a - d_in.addr) for a in ins]
     ctx.set_waves_per_block(wpb)
+    ctx.set_team(2 if team else 1)
     cl, st = ctx.compress_many_device(ins, lens, params, cs, caps)
     kc = ctx.stats()["codec_ms"]
     dl, dst = ctx.decompress_many_device(cs, cl, outs, lens, spec)
@@ -86,7 +90,7 @@ def _measure(ctx, d_in, ins, lens, d_c, d_o, chunk_max, table_bits, spec, wpb, c
         raise RuntimeError("autotune trial failed")
     total = sum(lens)
     return {"chunk_max": chunk_max, "table_bits": table_bits, "spec_bits": spec,
-            "waves_per_block": wpb, "ratio": total / float(sum(cl)),
+            "waves_per_block": wpb, "team": team, "ratio": total / float(sum(cl)),
             "compress_ms": kc, "decompress_ms": kd,
             "score": total / ((kc + kd) / 1e3)}
 
@@ -95,7 +99,7 @@ def autotune_batch(ctx, arena, offs, sizes, sample_bytes=96 << 20, ratio_tol=0.0
                    log=None):
     """Tunes the batch settings on a sample of the files (arena/offs/sizes as
     datagen.markov2_files returns them).  Returns a dict with chunk_max,
-    table_bits, spec_bits, waves_per_block and the trials."""
+    table_bits, spec_bits, waves_per_block, team and the trials."""
     key = "%d files, %d bytes, tol %.4f" % (len(sizes), int(np.sum(sizes)), ratio_tol)
     if cache and os.path.exists(cache):
         got = json.load(open(cache)).get(key)
@@ -128,16 +132,23 @@ def autotune_batch(ctx, arena, offs, sizes, sample_bytes=96 << 20, ratio_tol=0.0
         for wpb in WPB_GRID:
             launch.append(_measure(ctx, d_in, ins, lens, d_c, d_o, best["chunk_max"],
                                    best["table_bits"], spec, wpb, coffs))
+    for spec in TEAM_SPEC_GRID:
+        launch.append(_measure(ctx, d_in, ins, lens, d_c, d_o, best["chunk_max"],
+                               best["table_bits"], spec, 0, coffs, team=True))
     fast = max(launch, key=lambda t: t["score"])
     ctx.set_waves_per_block(0)
+    ctx.set_team(0)
     res = {"chunk_max": best["chunk_max"], "table_bits": best["table_bits"],
            "spec_bits": fast["spec_bits"], "waves_per_block": fast["waves_per_block"],
+           "team": fast["team"],
            "sample_files": len(idx), "sample_bytes": int(sum(lens)), "ratio_tol": ratio_tol,
            "seconds": time.time() - t0, "stream_trials": trials, "launch_trials": launch}
     if log:
-        log("autotune: chose chunk %d KiB, 2^%d slots, %d-bit speculation, %s waves/workgroup "
+        log("autotune: chose chunk %d KiB, 2^%d slots, %d-bit speculation, %s "
             "(%.1fs)" % (res["chunk_max"] // KiB, res["table_bits"], res["spec_bits"],
-                         res["waves_per_block"] or "auto", res["seconds"]))
+                         "teams of 4 waves" if res["team"] else
+                         "%s waves/workgroup" % (res["waves_per_block"] or "auto"),
+                         res["seconds"]))
     if cache:
         d = json.load(open(cache)) if os.path.exists(cache) else {}
         d[key] = res

@@ -401,6 +401,7 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
   if (sh.slots() == 0) return CUCM_E_BUDGET;
   CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
   st.waves = std::max<uint64_t>(st.waves, sh.waves());
+  if (sh.team) st.teams = std::max<uint64_t>(st.teams, sh.blocks);
   st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
@@ -709,6 +710,7 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   if (sh.slots() == 0) return CUCM_E_BUDGET;
   CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
   st.waves = std::max<uint64_t>(st.waves, sh.waves());
+  if (sh.team) st.teams = std::max<uint64_t>(st.teams, sh.blocks);
   st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
@@ -1002,6 +1004,7 @@ int cucm_compress_plain_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const d
     acc.tasks += ctx->last.tasks;
     acc.launches += ctx->last.launches;
     acc.waves = std::max(acc.waves, ctx->last.waves);
+    acc.teams = std::max(acc.teams, ctx->last.teams);
     acc.in_bytes += ctx->last.in_bytes;
     acc.out_bytes += ctx->last.out_bytes;
     acc.model_bytes = std::max(acc.model_bytes, ctx->last.model_bytes);
@@ -1028,6 +1031,7 @@ int cucm_decompress_plain_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const
     acc.tasks += ctx->last.tasks;
     acc.launches += ctx->last.launches;
     acc.waves = std::max(acc.waves, ctx->last.waves);
+    acc.teams = std::max(acc.teams, ctx->last.teams);
     acc.in_bytes += ctx->last.in_bytes;
     acc.out_bytes += ctx->last.out_bytes;
     acc.model_bytes = std::max(acc.model_bytes, ctx->last.model_bytes);

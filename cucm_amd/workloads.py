@@ -52,6 +52,7 @@ def batch_workload(args):
         tuned = autotune_batch(ctx, arena, offs, sizes, log=log)
         s_max, tb, spec = tuned["chunk_max"], tuned["table_bits"], tuned["spec_bits"]
         ctx.set_waves_per_block(tuned["waves_per_block"])
+        ctx.set_team(2 if tuned.get("team") else 1)
     params = [make_params(pick_chunk_size(int(n), s_max), 128, 100, 0, tb) for n in sizes]
     d_in = ctx.alloc(total)
     d_in.upload(arena)
@@ -110,7 +111,8 @@ def batch_workload(args):
                                    count, tb, s_max // 1024, ", autotuned" if tuned else ""),
                    "files": count, "bytes": total, "table_bits": tb, "chunk_max": s_max,
                    "spec_bits": spec,
-                   "waves_per_block": tuned["waves_per_block"] if tuned else 0},
+                   "waves_per_block": tuned["waves_per_block"] if tuned else 0,
+                   "team": bool(tuned.get("team")) if tuned else False},
         "batched_compress_MBps": total / 1e6 / (b - a),
         "batched_decompress_MBps": total / 1e6 / (c - b),
         "batched_compress_kernel_ms": sc["codec_ms"],
@@ -124,10 +126,11 @@ def batch_workload(args):
     }
     if tuned:
         res["autotune"] = {k: tuned[k] for k in ("chunk_max", "table_bits", "spec_bits",
-                                                 "waves_per_block", "sample_files",
+                                                 "waves_per_block", "team", "sample_files",
                                                  "sample_bytes", "ratio_tol", "seconds")}
-        res["autotune"]["trials"] = [{k: t[k] for k in ("chunk_max", "table_bits", "spec_bits",
-                                                        "waves_per_block", "ratio", "score")}
+        res["autotune"]["trials"] = [{k: t.get(k) for k in ("chunk_max", "table_bits",
+                                                            "spec_bits", "waves_per_block",
+                                                            "team", "ratio", "score")}
                                      for t in tuned["stream_trials"] + tuned["launch_trials"]]
     res["batched_vs_plain_compress"] = res["batched_compress_MBps"] / res["plain_compress_MBps"]
     res["batched_vs_plain_decompress"] = res["batched_decompress_MBps"] / res["plain_decompress_MBps"]

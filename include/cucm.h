@@ -178,6 +178,7 @@ typedef struct cucm_stats {
   uint64_t launches;       /* codec kernel launches */
   uint64_t in_bytes, out_bytes;
   uint64_t model_bytes;    /* HBM held by model slots */
+  uint64_t teams;          /* workgroups that ran as teams of 4 waves (0: one wave per chunk) */
 } cucm_stats;
 int cucm_ctx_last_stats(cucm_ctx *ctx, cucm_stats *st);
 
