@@ -215,13 +215,16 @@ uint64_t team_capacity(const cucm_ctx *c, uint32_t Umax, uint32_t spec, bool dec
 }
 
 // Teams run when forced on, or (automatic) when every chunk of the launch
-// can be resident as a team at once: then a chunk's latency, not the chip's
-// memory traffic, bounds the launch, and the team's extra waves shorten it.
+// can be resident as a team at once -- then a chunk's latency bounds the
+// launch and the team's extra waves shorten it -- or when the chunks fill at
+// least two full passes of resident teams (measured: 4096 x 256 KiB chunks
+// code 6% faster as teams than as single waves; 1024 chunks, 1.33 passes of
+// teams, 12% slower because of the ragged last pass).
 bool use_team(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, bool dec) {
   if (c->team_mode == 1 || !team_supported(dec, spec)) return false;
   const uint64_t cap = team_capacity(c, Umax, spec, dec);
   if (cap == 0) return false;
-  return c->team_mode == 2 || ntasks <= cap;
+  return c->team_mode == 2 || ntasks <= cap || ntasks >= 2 * cap;
 }
 
 Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, bool dec,

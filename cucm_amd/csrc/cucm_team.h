@@ -664,11 +664,12 @@ CUCM_DEV void decode_team(const CodecArgs &a, const Task &tk, uint32_t t, const 
 // Team encoder: one chunk per workgroup of kTeamWaves waves.  Per unit u:
 //   P1  waves 0, 1 stage, hash, gather and insert unit u (half the bits
 //       each); wave 2 walks unit u - 1's statep recurrences; wave 3 codes
-//       unit u - 1's bits;
+//       the second half of unit u - 1's bits;
 //   P2  waves 0, 1 predict unit u, keeping each bit's mixer deltas in
 //       registers;
 //   P3  waves 0, 1 add the deltas; wave 1 resolves the unit's states byte by
-//       byte; wave 2 keeps the unit's bytes for its walk;
+//       byte; wave 2 keeps the unit's bytes for its walk; wave 3 codes the
+//       first half of unit u's bits;
 //   P4  all four write the touched slots back and clear the tables (a full
 //       barrier: the write-backs land before unit u + 1's gathers).
 // The model is updated in place: nothing reads statep during P1 but the
@@ -894,12 +895,14 @@ CUCM_DEV void team_enc_issuer(const Task &tk, const TeamMem &X, const LDS DevTab
                         sl, g, idx, r0);
     clk.mark(2);
     team_sync_lds();  // P1
+    clk.mark(9);
     if (!nvalid) continue;
     int32_t dl[NJW][8];
     uint32_t cx[NJW];
     encode_predict_team<NJW>(M, T, nvalid, Umax, lane, r0, sl, g, idx, dl, cx);
     clk.mark(3);
     team_sync_lds();  // P2
+    clk.mark(9);
 #pragma unroll
     for (int j = 0; j < NJW; ++j)
       if (r0 + lane + 64u * j < nvalid)
@@ -909,13 +912,16 @@ CUCM_DEV void team_enc_issuer(const Task &tk, const TeamMem &X, const LDS DevTab
     if (w == 1) team_resolve_unit(X, T, nvalid / 8, Umax, lane);
     clk.mark(6);
     team_sync_lds();  // P3
+    clk.mark(9);
     team_enc_writeback(X, slot, tb, nvalid, Umax, w, lane);
-    team_sync();  // P4
     clk.mark(8);
+    team_sync();  // P4
+    clk.mark(9);
   }
 }
 
-// Wave 2: walks unit u - 1 during P1 of unit u.
+// Wave 2: walks unit u - 1 during P1 of unit u (restarting at every byte,
+// which measures faster than one continuous walk here).
 CUCM_DEV void team_enc_walker(const Task &tk, const TeamMem &X, const LDS DevTables *T,
                               GLB uint8_t *slot, uint32_t Umax, uint32_t lane, PhaseClock &clk) {
   const WaveMem &M = X.M;
@@ -926,13 +932,17 @@ CUCM_DEV void team_enc_walker(const Task &tk, const TeamMem &X, const LDS DevTab
     for (uint32_t b = 0; b < prev / 8; ++b) CUCM_TWALK_BYTE(X, T, b, X.ubw[b], Umax, lane);
     clk.mark(7);
     team_sync_lds();  // P1
+    clk.mark(10);
     const uint32_t nvalid = E.nvalid(u);
     if (!nvalid) continue;
     team_sync_lds();  // P2
     for (uint32_t i = lane; i < nvalid / 8; i += 64) X.ubw[i] = M.ub[8 + i];
     team_sync_lds();  // P3
+    clk.mark(10);
     team_enc_writeback(X, slot, tk.tb, nvalid, Umax, 2, lane);
+    clk.mark(8);
     team_sync();  // P4
+    clk.mark(10);
   }
 }
 
@@ -952,21 +962,29 @@ CUCM_DEV void team_enc_coder(const CodecArgs &a, const Task &tk, uint32_t t, con
       uint32_t pv[NJ];
 #pragma unroll
       for (int j = 0; j < NJ; ++j) pv[j] = M.rprm[lane + 64u * j];
+      // coded bits are a multiple of 8: blocks of 8 with constant lane offsets
       const uint32_t nc = E.ncoded(u - 1);
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
         const uint32_t lo = 64u * j, hi = min(nc, lo + 64u);
-        for (uint32_t r = lo; r < hi; ++r) CUCM_CODE_BIT(lane_read(pv[j], r - lo));
+        for (uint32_t r0 = lo; r0 < hi; r0 += 8) {
+#pragma unroll
+          for (uint32_t t8 = 0; t8 < 8; ++t8) CUCM_CODE_BIT(lane_read(pv[j], r0 - lo + t8));
+        }
       }
     }
     clk.mark(4);
     team_sync_lds();  // P1
+    clk.mark(11);
     const uint32_t nvalid = E.nvalid(u);
     if (!nvalid) continue;
     team_sync_lds();  // P2
     team_sync_lds();  // P3
+    clk.mark(11);
     team_enc_writeback(X, slot, tk.tb, nvalid, Umax, 3, lane);
+    clk.mark(8);
     team_sync();  // P4
+    clk.mark(11);
   }
   acc |= (x1 >> 24) << (8 * (opos & 3u));
   ++opos;

@@ -96,6 +96,9 @@ CUCM_DEV void flag_publish(LDS uint32_t *p, uint32_t v) {
 CUCM_DEV uint32_t flag_read(const LDS uint32_t *p) {
   return __hip_atomic_load((uint32_t *)p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// Low 32 bits of a * b for operands that fit 24-bit signed integers (a
+// full-rate multiply; the 32-bit product is quarter rate).
+CUCM_DEV uint32_t mul_i24(int32_t a, int32_t b) { return (uint32_t)__mul24(a, b); }
 CUCM_DEV uint32_t lds_cas(LDS uint32_t *p, uint32_t cmp, uint32_t v) {
   return atomicCAS((uint32_t *)p, cmp, v);
 }
@@ -493,7 +496,8 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
     const uint32_t wdfwd_ = w_dtab[wnn_];                                                  \
     const uint32_t wdraw_ = w_dtab[w_raw1 & 1023u];                                        \
     const int32_t we_ = (int32_t)(((uint32_t)(y) << 22) - (w_p >> 10));                   \
-    const uint32_t wpn_ = w_p + (((uint32_t)(we_ >> 3) * wd_) & 0xfffffc00u) + (wnn_ - wn_); \
+    /* |we_ >> 3| <= 2^19 and wd_ <= 5461: a 24-bit product */                          \
+    const uint32_t wpn_ = w_p + (mul_i24(we_ >> 3, (int32_t)wd_) & 0xfffffc00u) + (wnn_ - wn_); \
     *(w_writer ? &w_sp[w_s0] : w_dummy) = wpn_;                                            \
     const bool wsame1_ = w_s1 == w_s0, wsame2_ = w_s1 == w_sprev;                          \
     w_p = wsame1_ ? wpn_ : (wsame2_ ? w_pnprev : w_raw1);                                  \

@@ -207,7 +207,8 @@ int cucm_ctx_set_waves_per_block(cucm_ctx *ctx, uint32_t waves);
  * the StateMap recurrences and hash the next round's paths.  mode 0
  * (default): teams when every chunk of a launch can be resident as a team at
  * once (decoder: spec_bits 1, 2 or 4); 1: never; 2: whenever allowed.  Never
- * changes the output. */
+ * changes the output.  Automatic mode also uses teams when the chunks fill
+ * at least two full passes of resident teams. */
 int cucm_ctx_set_team(cucm_ctx *ctx, uint32_t mode);
 
 /* ---- measurement probes (device 0 / current device) ------------------- */
