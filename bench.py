@@ -143,6 +143,7 @@ def stream_workload(args):
     roofline = {"bound": "hbm", "kernel": kname, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                 "traffic": measured_traffic(kname, n)}
+    roofline.update(physical_rates(kname, n, kms / 1000.0))
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -172,6 +173,28 @@ def measured_traffic(kernel, n):
     if t is None:
         return None
     return (t["fetch_bytes_per_input_byte"] + t["write_bytes_per_input_byte"]) * n
+
+
+def physical_rates(kernel, n, seconds):
+    """The dominant kernel's physical HBM rate (measured bytes per input byte
+    x input bytes / kernel time), its L2 hit rate on the state gathers, and
+    its 128-byte line fetches per second as a fraction of the measured
+    random one-byte read ceiling on the same footprint."""
+    path = os.path.join(ROOT, "profiles", "codec_kernel_traffic.json")
+    if not os.path.exists(path):
+        return {}
+    d = json.load(open(path))
+    t, ceil = d.get(kernel), d.get("gather_ceiling")
+    if t is None or ceil is None:
+        return {}
+    fetch = t["fetch_bytes_per_input_byte"] * n
+    write = t["write_bytes_per_input_byte"] * n
+    lines_per_s = fetch / ceil["line_bytes"] / seconds
+    return {"hbm_GBps": (fetch + write) / seconds / 1e9,
+            "hbm_frac_of_peak": (fetch + write) / seconds / 1e9 / HBM_PEAK_GBS,
+            "l2_hit_rate": t["l2_hit_rate"],
+            "line_fetches_per_s": lines_per_s,
+            "gather_ceiling_frac": lines_per_s / ceil["read_touches_per_s"]}
 
 
 def chunk_payloads(container, nchunks_max):
