@@ -19,6 +19,10 @@ chunks, 2^22 slots), --workload small (1 MiB order-2 Markov, 16 KiB chunks,
 2^18 slots), --workload batch (10,000 lognormal files, batched vs plain
 launch), --workload serve (Poisson request mix, p50/p99), --workload gather
 (random-touch ceiling), --workload copy (counter calibration).
+
+--frozen runs a single-stream workload with the frozen model (the initial
+model, trained on the first --prelearn-kib KiB of the input, codes every
+chunk without updates; CUCM_FLAG_FROZEN).
 """
 import argparse
 import json
@@ -38,6 +42,8 @@ HBM_PEAK_GBS = 8000.0
 # gathers + the same number of one-byte state write-backs, 1 B of input and
 # ~1/ratio B of output (order-0 and statep/mixer live in LDS).
 ALG_BYTES_PER_INPUT_BYTE = 128.0
+# The frozen model reads the same 64 states per input byte and writes none.
+ALG_BYTES_PER_INPUT_BYTE_FROZEN = 64.0
 
 
 def log(*a):
@@ -66,13 +72,16 @@ def stream_workload(args):
     mib = args.mib or mib_default
     n = mib * MiB
     U, batch = 128, 100
+    P = args.prelearn_kib * 1024 if args.prelearn_kib >= 0 else (MiB if args.frozen else 0)
+    if P >= n:
+        raise SystemExit("prelearn must be smaller than the input")
     t0 = time.time()
     seed = {"mixed": 2, "structured": 4, "markov2": 1}[gen]
     data = getattr(datagen, gen)(n, seed)
     log("generated %d MiB %s input in %.1fs" % (mib, gen, time.time() - t0))
     ctx = cucm_amd.Context()
-    params = cucm_amd.make_params(S, U, batch, 0, tb)
-    C = (n + S - 1) // S
+    params = cucm_amd.make_params(S, U, batch, P, tb, frozen=args.frozen)
+    C = (n - P + S - 1) // S
     cap = n + n // 2 + 4096 * (C + 1)
     d_in = ctx.alloc(n)
     d_in.upload(data)
@@ -137,24 +146,35 @@ def stream_workload(args):
     # launch ran as teams of four waves, else the one-wave-per-chunk kernel)
     dom = "decode" if res["decompress_kernel_ms"] >= res["compress_kernel_ms"] else "encode"
     teams = res["decompress_teams"] if dom == "decode" else res["compress_teams"]
-    kname = ("%s_team_kernel" % dom) if teams else ("codec_kernel<%s>" % dom)
+    if args.frozen:
+        kname = "frozen_kernel<%s>" % dom
+    else:
+        kname = ("%s_team_kernel" % dom) if teams else ("codec_kernel<%s>" % dom)
     kms = max(res["compress_kernel_ms"], res["decompress_kernel_ms"])
-    achieved = ALG_BYTES_PER_INPUT_BYTE * n / (kms / 1000.0) / 1e9
+    alg = ALG_BYTES_PER_INPUT_BYTE_FROZEN if args.frozen else ALG_BYTES_PER_INPUT_BYTE
+    achieved = alg * (n - P) / (kms / 1000.0) / 1e9
     roofline = {"bound": "hbm", "kernel": kname, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(kname, n)}
-    roofline.update(physical_rates(kname, n, kms / 1000.0))
+                "traffic": measured_traffic(kname, n - P)}
+    roofline.update(physical_rates(kname, n - P, kms / 1000.0))
+    res["compress_kernel_MBps"] = (n - P) / 1e6 / (res["compress_kernel_ms"] / 1000.0)
+    res["decompress_kernel_MBps"] = (n - P) / 1e6 / (res["decompress_kernel_ms"] / 1000.0)
     cpu = None
     parity = None
     if not args.no_cpu:
-        cpu, parity = cpu_baseline(data, container, S, U, tb, args)
+        cpu, parity = cpu_baseline(data, container, S, U, tb, P, args)
     line = {
         "metric": METRIC, "value": value, "unit": "MB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32 fixed-point",
         "data": "synthetic (seeded generator, resident in HBM)",
-        "config": {"workload": desc % mib, "chunks": C, "chunk_size": S, "unit_bits": U,
-                   "table_bits": tb, "batch_chunks": batch, "spec_bits": args.spec},
+        "config": {"workload": (desc % mib) if not (args.frozen or P) else
+                   (desc % mib).replace("P=0", "P=%d KiB%s" % (P // 1024,
+                                                               ", frozen model" if args.frozen
+                                                               else "")),
+                   "chunks": C, "chunk_size": S, "unit_bits": U, "table_bits": tb,
+                   "batch_chunks": batch, "spec_bits": args.spec, "prelearn_bytes": P,
+                   "frozen": bool(args.frozen)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     line.update(res)
@@ -202,7 +222,7 @@ def chunk_payloads(container, nchunks_max):
     import struct
     P = struct.unpack(">I", container[16:20])[0]
     Pc = struct.unpack(">I", container[12:16])[0]
-    flag = container[20]
+    flag = container[20] & 1  # bit 0: coded prelearn segment (bit 1: frozen model)
     N = struct.unpack(">I", container[0:4])[0]
     batch = struct.unpack(">H", container[4:6])[0]
     S = struct.unpack(">I", container[8:12])[0]
@@ -226,19 +246,28 @@ def chunk_payloads(container, nchunks_max):
     return out
 
 
-def cpu_baseline(data, container, S, U, tb, args):
+def cpu_baseline(data, container, S, U, tb, P, args):
     """Times the oracle (a port of the reference model) on a bounded sample of
-    the same workload: `sample` whole chunks, one chunk per core (OpenMP)."""
+    the same workload: `sample` whole chunks after the P prelearn bytes, one
+    chunk per core (OpenMP), from the same initial model."""
     from oracle import oracle as O
 
     threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-    k = min(args.cpu_chunks or threads, (len(data) + S - 1) // S)
-    sample = data[: k * S].tobytes()
+    k = min(args.cpu_chunks or threads, (len(data) - P + S - 1) // S)
+    sample = data[P: P + k * S].tobytes()
     model = O.Model(tb)
+    if P:
+        O.prelearn_train(data[:P].tobytes(), model)
     t0 = time.perf_counter()
-    blob = O.encode_batch(sample, S, U, model, tail_pad=True, threads=threads)
+    if args.frozen:
+        blob = O.encode_batch_frozen(sample, S, model, threads=threads)
+    else:
+        blob = O.encode_batch(sample, S, U, model, tail_pad=True, threads=threads)
     t1 = time.perf_counter()
-    back = O.decode_batch(blob, len(sample), S, U, model, threads=threads)
+    if args.frozen:
+        back = O.decode_batch_frozen(blob, len(sample), S, model, threads=threads)
+    else:
+        back = O.decode_batch(blob, len(sample), S, U, model, threads=threads)
     t2 = time.perf_counter()
     ok = back == sample
     # compare the oracle's payloads with the GPU container's first k chunks
@@ -253,9 +282,10 @@ def cpu_baseline(data, container, S, U, tb, args):
         prev = e
     cpu = {"value": len(sample) / 1e6 / (t1 - t0), "unit": "MB/s (compress)",
            "decompress_MBps": len(sample) / 1e6 / (t2 - t1), "cores": threads, "kind": "port",
-           "sample": "%d chunks x %d KiB of the same input (its first %d KiB), oracle "
-                     "encode_batch/decode_batch, OpenMP one chunk per core" % (k, S // 1024,
-                                                                              len(sample) // 1024),
+           "sample": "%d chunks x %d KiB of the same input (its first %d KiB after the "
+                     "prelearn bytes), oracle encode_batch/decode_batch%s, OpenMP one chunk per "
+                     "core" % (k, S // 1024, len(sample) // 1024,
+                               "_frozen" if args.frozen else ""),
            "lossless": ok}
     return cpu, same
 
@@ -269,6 +299,11 @@ def main():
                     choices=["stream", "structured", "small", "batch", "serve", "gather", "copy"])
     ap.add_argument("--mib", type=int, default=0, help="input size (0 = the workload's own)")
     ap.add_argument("--spec", type=int, default=4)
+    ap.add_argument("--frozen", action="store_true",
+                    help="single-stream workloads: code with the frozen model")
+    ap.add_argument("--prelearn-kib", type=int, default=-1,
+                    help="single-stream workloads: prelearn bytes (default 1024 KiB with "
+                         "--frozen, else 0)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-chunks", type=int, default=0)

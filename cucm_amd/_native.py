@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libcucm.so")
 OK = 0
 E_PARAM, E_CORRUPT, E_SMALL, E_NOMEM, E_BUDGET, E_DEVICE, E_UNSUPPORTED, E_NOTREADY = range(-1, -9, -1)
 FLAG_NO_TAIL_PAD = 0x1
+FLAG_FROZEN = 0x2  # frozen model: the initial model predicts every bit, never updated
 OP_COMPRESS, OP_DECOMPRESS = 0, 1
 
 # Symbols declared in include/cucm.h (checked by the test suite).

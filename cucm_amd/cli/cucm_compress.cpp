@@ -3,7 +3,7 @@
 // input goes through the streaming C ABI one segment of batch records at a
 // time, so files larger than the GPU's memory compress with a bounded
 // footprint.  Extra options: -t/--table-bits (default 25 = reference
-// containers), -d/--device, --no-tail-pad.
+// containers), -d/--device, --no-tail-pad, --frozen.
 #include <chrono>
 #include <cstring>
 #include <iostream>
@@ -26,6 +26,8 @@ static void print_help(const char *prog) {
   std::cout << "  -t, --table-bits <N>  log2 slots per order table, 8..25 (default: 25)\n";
   std::cout << "  -d, --device <N>      GPU to use (default: 0)\n";
   std::cout << "      --no-tail-pad     Code the last chunk only up to its end\n";
+  std::cout << "      --frozen          Frozen model: the prelearned (or zero) model codes every\n"
+               "                        chunk and is never updated\n";
   std::cout << "  -h, --help            Show this help message\n";
 }
 
@@ -40,7 +42,7 @@ int main(int argc, char *argv[]) {
   }
   int unit_size = 128, chunknum = -1, batch = 100, table_bits = 25, device = 0;
   uint64_t chunk_size = 1024 * 1024, prelearn_size = 0;
-  bool tail_pad = true;
+  bool tail_pad = true, frozen = false;
   std::string filename = "./enwik8", output_filename = "./compressed.bin";
   for (int i = 1; i < argc; ++i) {
     const bool more = i + 1 < argc;
@@ -81,6 +83,8 @@ int main(int argc, char *argv[]) {
       device = std::atoi(argv[++i]);
     } else if (is(argv[i], "--no-tail-pad", nullptr)) {
       tail_pad = false;
+    } else if (is(argv[i], "--frozen", nullptr)) {
+      frozen = true;
     } else if (is(argv[i], "-h", "--help")) {
       print_help(argv[0]);
       return 0;
@@ -124,7 +128,7 @@ int main(int argc, char *argv[]) {
   p.batch_chunks = (uint16_t)batch;
   p.prelearn_size = (uint32_t)prelearn_size;
   p.table_bits = (uint8_t)table_bits;
-  p.flags = tail_pad ? 0u : CUCM_FLAG_NO_TAIL_PAD;
+  p.flags = (tail_pad ? 0u : CUCM_FLAG_NO_TAIL_PAD) | (frozen ? CUCM_FLAG_FROZEN : 0u);
   cucm_ctx *ctx = nullptr;
   int rc = cucm_ctx_create(&ctx, device, 0);
   if (rc) {

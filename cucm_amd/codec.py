@@ -11,13 +11,15 @@ import ctypes
 
 import numpy as np
 
-from ._native import (FLAG_NO_TAIL_PAD, CucmError, Job, Params, Result, Stats, check, lib)
+from ._native import (FLAG_FROZEN, FLAG_NO_TAIL_PAD, CucmError, Job, Params, Result, Stats, check,
+                      lib)
 
 MiB = 1 << 20
 
 
 def make_params(chunk_size=MiB, unit_bits=128, batch_chunks=100, prelearn_size=0, table_bits=25,
-                spec_bits=4, tail_pad=True):
+                spec_bits=4, tail_pad=True, frozen=False):
+    """cucm_params; frozen=True selects the frozen model (CUCM_FLAG_FROZEN)."""
     p = Params()
     lib().cucm_default_params(ctypes.byref(p))
     p.chunk_size = chunk_size
@@ -26,7 +28,7 @@ def make_params(chunk_size=MiB, unit_bits=128, batch_chunks=100, prelearn_size=0
     p.prelearn_size = prelearn_size
     p.table_bits = table_bits
     p.spec_bits = spec_bits
-    p.flags = 0 if tail_pad else FLAG_NO_TAIL_PAD
+    p.flags = (0 if tail_pad else FLAG_NO_TAIL_PAD) | (FLAG_FROZEN if frozen else 0)
     return p
 
 
@@ -283,7 +285,8 @@ class Context:
                                                ctypes.byref(s)), "stream_compress_begin")
         parts = [head[: hl.value].tobytes()]
         q = make_params(params.chunk_size, params.unit_bits, params.batch_chunks, 0,
-                        params.table_bits, tail_pad=not (params.flags & FLAG_NO_TAIL_PAD))
+                        params.table_bits, tail_pad=not (params.flags & FLAG_NO_TAIL_PAD),
+                        frozen=bool(params.flags & FLAG_FROZEN))
         seg = params.batch_chunks * params.chunk_size * max(1, segment_batches)
         try:
             pos = P
@@ -391,8 +394,9 @@ def default_context():
 
 
 def compress(data, chunk_size=MiB, unit_bits=128, batch_chunks=100, prelearn_size=0, table_bits=25,
-             tail_pad=True) -> bytes:
-    p = make_params(chunk_size, unit_bits, batch_chunks, prelearn_size, table_bits, 4, tail_pad)
+             tail_pad=True, frozen=False) -> bytes:
+    p = make_params(chunk_size, unit_bits, batch_chunks, prelearn_size, table_bits, 4, tail_pad,
+                    frozen)
     return default_context().compress(data, p)
 
 

@@ -125,7 +125,8 @@ int PinnedBuf::reserve(uint64_t n) {
 
 // The 21-byte container header (cucm_compress_v2.cu:1172-1224, format
 // comment :1420-1443): BE u32 N, BE u16 batch chunks, BE u16 U with the
-// table bits in its low 5 bits (0 = 25), BE u32 S, Pc, P, the flag byte.
+// table bits in its low 5 bits (0 = 25), BE u32 S, Pc, P, the flag byte
+// (bit 0: the prelearn segment is coded; bit 1, kHeaderFrozen: frozen model).
 void write_header(uint8_t *h, uint64_t N, const cucm_params &p, uint32_t Pc, uint32_t flag) {
   put_be32(h, (uint32_t)N);
   h[4] = (uint8_t)(p.batch_chunks >> 8);
@@ -136,7 +137,7 @@ void write_header(uint8_t *h, uint64_t N, const cucm_params &p, uint32_t Pc, uin
   put_be32(h + 8, p.chunk_size);
   put_be32(h + 12, Pc);
   put_be32(h + 16, p.prelearn_size);
-  h[20] = (uint8_t)flag;
+  h[20] = (uint8_t)(flag | ((p.flags & CUCM_FLAG_FROZEN) ? kHeaderFrozen : 0u));
 }
 
 // ---------------------------------------------------------------------------
@@ -262,6 +263,47 @@ Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spe
   return s;
 }
 
+// Frozen-model tasks: one wave per chunk, workgroups of up to 8 waves
+// sharing the tables, as many resident as LDS allows; no model slots (the
+// read-only model is read in place).
+Shape frozen_shape(const cucm_ctx *c, uint64_t ntasks) {
+  Shape s;
+  uint32_t wmax = 8;
+  while (wmax > 1 && frozen_lds_bytes(wmax) > c->lds_block) --wmax;
+  const uint64_t per_cu = std::max<uint64_t>(1, c->lds_cu / frozen_lds_bytes(wmax));
+  uint64_t waves = std::min<uint64_t>(ntasks, (uint64_t)c->cus * per_cu * wmax);
+  if (c->max_waves) waves = std::min<uint64_t>(waves, c->max_waves);
+  if (waves == 0) return s;
+  uint32_t w = (uint32_t)std::min<uint64_t>(wmax, (waves + c->cus - 1) / c->cus);
+  if (c->waves_per_block) w = std::min<uint32_t>(c->waves_per_block, wmax);
+  s.wpb = std::max<uint32_t>(w, 1);
+  s.blocks = (uint32_t)((waves + s.wpb - 1) / s.wpb);
+  return s;
+}
+
+// One codec pass over the tasks on the device: the unit codec (shape sh)
+// for the tasks without kTaskFrozen and the frozen codec (shape fz) for the
+// rest, back to back on the context's stream, each kernel pulling from its
+// own queue head (counter words 0 and 1) and skipping the other's tasks.
+// With `timed`, events 0 and 1 bracket both launches.
+int launch_pass(cucm_ctx *c, const CodecArgs &a, bool decode, uint32_t Umax, const Shape &sh,
+                const Shape &fz, bool timed, cucm_stats &st) {
+  HIP_TRY(hipMemsetAsync(c->counter.p, 0, 8, c->stream));
+  if (timed) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  if (sh.blocks) {
+    HIP_TRY(launch_codec(decode, a, Umax, sh.blocks, sh.wpb, sh.team, c->stream));
+    st.launches += 1;
+  }
+  if (fz.blocks) {
+    CodecArgs f = a;
+    f.counter = a.counter + 1;
+    HIP_TRY(launch_frozen(decode, f, fz.blocks, fz.wpb, c->stream));
+    st.launches += 1;
+  }
+  if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+  return CUCM_OK;
+}
+
 struct InitModels {
   std::vector<DevInitModel> hostside;
   std::vector<std::unique_ptr<DevBuf>> states;
@@ -355,7 +397,8 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
       if (!seg.empty()) std::memcpy(J.header.data() + kHeader, seg.data(), seg.size());
     }
     J.first = (uint32_t)tasks.size();
-    const bool pad = !(J.p.flags & CUCM_FLAG_NO_TAIL_PAD);
+    const bool frozen = (J.p.flags & CUCM_FLAG_FROZEN) != 0;
+    const bool pad = !(J.p.flags & CUCM_FLAG_NO_TAIL_PAD) && !frozen;
     for (uint32_t ci = 0; ci < J.C; ++ci) {
       Task t{};
       t.src = (uint64_t)(uintptr_t)(J.din + P + (uint64_t)ci * S);
@@ -364,7 +407,7 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
       t.U = J.p.unit_bits;
       t.tb = J.p.table_bits;
       t.flags = (uint8_t)(((pad && ci == J.C - 1) ? kTaskTailPad : 0) |
-                          (J.model >= 0 ? kTaskInitModel : 0));
+                          (J.model >= 0 ? kTaskInitModel : 0) | (frozen ? kTaskFrozen : 0));
       t.model = J.model >= 0 ? (uint32_t)J.model : 0;
       t.dst = staging_total;  // offset until the staging buffer is placed
       const uint64_t cap = staging_cap(S);
@@ -400,12 +443,18 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
   const uint64_t used = c->staging.cap + c->tasks.cap + c->results.cap + c->arena_in.cap +
                         c->arena_out.cap;
   const uint64_t mem_left = c->budget > used ? c->budget - used : 0;
-  Shape sh = pick_shape(c, ntasks, Umax, 0, false, slot_bytes, mem_left);
-  if (sh.slots() == 0) return CUCM_E_BUDGET;
-  CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
-  st.waves = std::max<uint64_t>(st.waves, sh.waves());
-  if (sh.team) st.teams = std::max<uint64_t>(st.teams, sh.blocks);
-  st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
+  uint32_t nfrozen = 0;
+  for (const Task &t : tasks) nfrozen += (t.flags & kTaskFrozen) ? 1u : 0u;
+  Shape sh, fz;
+  if (ntasks > nfrozen) {
+    sh = pick_shape(c, ntasks - nfrozen, Umax, 0, false, slot_bytes, mem_left);
+    if (sh.slots() == 0) return CUCM_E_BUDGET;
+    CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
+    if (sh.team) st.teams = std::max<uint64_t>(st.teams, sh.blocks);
+    st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
+  }
+  if (nfrozen) fz = frozen_shape(c, nfrozen);
+  st.waves = std::max<uint64_t>(st.waves, sh.waves() + fz.waves());
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
   a.ntasks = ntasks;
@@ -417,11 +466,7 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
   a.tabs = (const DevTables *)c->tabs.p;
   a.init = (const DevInitModel *)c->initm.p;
   CU_TRY(prof_arm(c, a));
-  HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
-  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  HIP_TRY(launch_codec(false, a, Umax, sh.blocks, sh.wpb, sh.team, c->stream));
-  HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-  st.launches += 1;
+  CU_TRY(launch_pass(c, a, false, Umax, sh, fz, true, st));
   CU_TRY(prof_collect(c, a));
   std::vector<uint64_t> res(ntasks);
   HIP_TRY(hipMemcpyAsync(res.data(), c->results.p, 8ull * ntasks, hipMemcpyDeviceToHost, c->stream));
@@ -457,11 +502,16 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
     b.tasks = (const Task *)c->tasks2.p;
     b.ntasks = (uint32_t)rt.size();
     b.result = (uint64_t *)c->results2.p;
-    Shape s2 = pick_shape(c, rt.size(), Umax, 0, false, slot_bytes, mem_left);
-    if (s2.slots() == 0) return CUCM_E_BUDGET;
-    HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
-    HIP_TRY(launch_codec(false, b, Umax, s2.blocks, s2.wpb, s2.team, c->stream));
-    st.launches += 1;
+    b.prof = nullptr;
+    uint32_t rfrozen = 0;
+    for (const Task &x : rt) rfrozen += (x.flags & kTaskFrozen) ? 1u : 0u;
+    Shape s2, f2;
+    if (rt.size() > rfrozen) {
+      s2 = pick_shape(c, rt.size() - rfrozen, Umax, 0, false, slot_bytes, mem_left);
+      if (s2.slots() == 0 || s2.slots() > sh.slots()) return CUCM_E_BUDGET;
+    }
+    if (rfrozen) f2 = frozen_shape(c, rfrozen);
+    CU_TRY(launch_pass(c, b, false, Umax, s2, f2, false, st));
     std::vector<uint64_t> r2(rt.size());
     HIP_TRY(hipMemcpyAsync(r2.data(), c->results2.p, 8 * rt.size(), hipMemcpyDeviceToHost,
                            c->stream));
@@ -572,7 +622,7 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   InitModels im;
   std::vector<ParseJob> pj;
   std::vector<uint32_t> pj_of;
-  uint32_t ntasks = 0, Umax = 32, tbmax = 8;
+  uint32_t ntasks = 0, nfrozen = 0, Umax = 32, tbmax = 8;
   for (uint32_t j = 0; j < n; ++j) {
     DecJob &J = jobs[j];
     J.status = CUCM_OK;
@@ -605,9 +655,10 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
       p.U = J.U;
       p.tb = J.tb;
       p.batch = J.batch;
-      p.flags = model >= 0 ? kTaskInitModel : 0;
+      p.flags = (model >= 0 ? kTaskInitModel : 0) | (J.frozen ? kTaskFrozen : 0);
       p.model = model >= 0 ? (uint32_t)model : 0;
       ntasks += p.ntasks;
+      if (J.frozen) nfrozen += p.ntasks;
       Umax = std::max(Umax, J.U);
       tbmax = std::max(tbmax, J.tb);
       pj.push_back(p);
@@ -623,11 +674,12 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
     const uint32_t batch = ((uint32_t)h[4] << 8) | h[5];
     const uint32_t uf = ((uint32_t)h[6] << 8) | h[7];
     const uint64_t S = get_be32(h + 8), Pc = get_be32(h + 12), P = get_be32(h + 16);
-    const uint32_t flag = h[20];
+    const uint32_t flag = h[20] & 1u;
+    const bool frozen = (h[20] & kHeaderFrozen) != 0;
     const uint32_t U = uf & ~31u;
     const uint32_t tb = (uf & 31u) ? (uf & 31u) : 25u;
     const uint64_t segb = flag ? Pc : P;
-    if (P > N || flag > 1 || tb < 8 || kHeader + segb > J.len) {
+    if (P > N || h[20] > (1u | kHeaderFrozen) || tb < 8 || kHeader + segb > J.len) {
       J.status = CUCM_E_CORRUPT;
       continue;
     }
@@ -681,9 +733,10 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
     p.U = U;
     p.tb = tb;
     p.batch = batch;
-    p.flags = model >= 0 ? kTaskInitModel : 0;
+    p.flags = (model >= 0 ? kTaskInitModel : 0) | (frozen ? kTaskFrozen : 0);
     p.model = model >= 0 ? (uint32_t)model : 0;
     ntasks += p.ntasks;
+    if (frozen) nfrozen += p.ntasks;
     Umax = std::max(Umax, U);
     tbmax = std::max(tbmax, tb);
     pj.push_back(p);
@@ -709,12 +762,16 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   const uint64_t used = c->staging.cap + c->tasks.cap + c->results.cap + c->arena_in.cap +
                         c->arena_out.cap;
   const uint64_t mem_left = c->budget > used ? c->budget - used : 0;
-  Shape sh = pick_shape(c, ntasks, Umax, spec, true, slot_bytes, mem_left);
-  if (sh.slots() == 0) return CUCM_E_BUDGET;
-  CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
-  st.waves = std::max<uint64_t>(st.waves, sh.waves());
-  if (sh.team) st.teams = std::max<uint64_t>(st.teams, sh.blocks);
-  st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
+  Shape sh, fz;
+  if (ntasks > nfrozen) {
+    sh = pick_shape(c, ntasks - nfrozen, Umax, spec, true, slot_bytes, mem_left);
+    if (sh.slots() == 0) return CUCM_E_BUDGET;
+    CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
+    if (sh.team) st.teams = std::max<uint64_t>(st.teams, sh.blocks);
+    st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
+  }
+  if (nfrozen) fz = frozen_shape(c, nfrozen);
+  st.waves = std::max<uint64_t>(st.waves, sh.waves() + fz.waves());
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
   a.ntasks = ntasks;
@@ -726,11 +783,7 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   a.tabs = (const DevTables *)c->tabs.p;
   a.init = (const DevInitModel *)c->initm.p;
   CU_TRY(prof_arm(c, a));
-  HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
-  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  HIP_TRY(launch_codec(true, a, Umax, sh.blocks, sh.wpb, sh.team, c->stream));
-  HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-  st.launches += 1;
+  CU_TRY(launch_pass(c, a, true, Umax, sh, fz, true, st));
   CU_TRY(prof_collect(c, a));
   std::vector<int32_t> stv(pj.size());
   HIP_TRY(hipMemcpyAsync(stv.data(), c->status.p, 4 * pj.size(), hipMemcpyDeviceToHost, c->stream));
@@ -830,6 +883,7 @@ int cucm_peek_header(const uint8_t *in, uint64_t len, uint64_t *orig, cucm_param
     out->table_bits = (uint8_t)((uf & 31u) ? (uf & 31u) : 25u);
     out->chunk_size = get_be32(in + 8);
     out->prelearn_size = get_be32(in + 16);
+    if (in[20] & kHeaderFrozen) out->flags |= CUCM_FLAG_FROZEN;
   }
   return CUCM_OK;
 }
@@ -1178,6 +1232,7 @@ int decompress_host(cucm_ctx *ctx, uint32_t n, const uint8_t *const in[], const 
       jobs[i].U = rec->U;
       jobs[i].tb = rec->tb;
       jobs[i].batch = rec->batch;
+      jobs[i].frozen = rec->frozen;
       jobs[i].ext_model = rec->ext_model;
     }
   }

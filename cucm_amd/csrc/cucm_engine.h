@@ -58,6 +58,9 @@ struct cucm_ctx {
   ~cucm_ctx();
 };
 
+// Bit 1 of the container's flag byte: chunks coded with the frozen model.
+constexpr uint32_t kHeaderFrozen = 2;
+
 struct EncJob {
   const uint8_t *din = nullptr;      // device input
   const uint8_t *host_in = nullptr;  // host copy of the input, if any
@@ -91,6 +94,7 @@ struct DecJob {
   bool records_only = false;
   uint64_t body = 0;
   uint32_t S = 0, U = 0, tb = 0, batch = 0;
+  bool frozen = false;
   const cucm::DevInitModel *ext_model = nullptr;
 };
 

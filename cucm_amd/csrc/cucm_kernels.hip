@@ -2,6 +2,7 @@
 // codec: the persistent codec kernel (its per-wave code is in cucm_wave.h),
 // container assembly, device-side container parsing, and the host launchers.
 #define CUCM_WITH_LAUNCHERS
+#include "cucm_frozen.h"
 #include "cucm_team.h"
 
 namespace cucm {
@@ -41,6 +42,7 @@ __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a) {
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
+    if (tk.flags & kTaskFrozen) continue;
     if constexpr (kDecode)
       decode_task<NJ, D>(a, tk, t, M, T, slot, Umax, lane);
     else
@@ -69,6 +71,7 @@ __global__ void __launch_bounds__(256) decode_team_kernel(CodecArgs a) {
     team_sync();
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
+    if (tk.flags & kTaskFrozen) continue;  // uniform: every wave read the same task
     decode_team<NJ, D>(a, tk, t, X, T, slot, Umax, wid, lane, rid);
   }
 }
@@ -92,7 +95,34 @@ __global__ void __launch_bounds__(256) encode_team_kernel(CodecArgs a) {
     team_sync();
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
+    if (tk.flags & kTaskFrozen) continue;  // uniform: every wave read the same task
     encode_team<NJ>(a, tk, t, X, T, slot, Umax, wid, lane);
+  }
+}
+
+// The frozen-model codec: one wave per chunk of the tasks marked
+// kTaskFrozen, the shared read-only model read in place (no model slots).
+template <bool kDecode, int D>
+__global__ void __launch_bounds__(512) frozen_kernel(CodecArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  LDS DevTables *T = (LDS DevTables *)lds;
+  stage_tables(a.tabs, T);
+  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const FrozenMem F =
+      carve_frozen((LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * frozen_wave_bytes());
+  uint32_t loaded = kNoModel;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.counter, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= a.ntasks) break;
+    const Task tk = a.tasks[t];
+    if (!(tk.flags & kTaskFrozen) || tk.L == 0) continue;
+    const GLB uint8_t *states = frozen_model(a, tk, F, T, loaded, lane);
+    if constexpr (kDecode)
+      frozen_decode_task<D>(a, tk, t, F, T, states, lane);
+    else
+      frozen_encode_task(a, tk, t, F, T, states, lane);
   }
 }
 
@@ -335,6 +365,34 @@ hipError_t launch_codec(bool decode, const CodecArgs &a, uint32_t Umax, uint32_t
   if (nj == 1) return launch_one<false, 1, 0>(a, Umax, blocks, waves_per_block, lds, stream);
   if (nj == 2) return launch_one<false, 2, 0>(a, Umax, blocks, waves_per_block, lds, stream);
   return launch_one<false, 4, 0>(a, Umax, blocks, waves_per_block, lds, stream);
+}
+
+uint32_t frozen_lds_bytes(uint32_t waves) {
+  return align16(sizeof(DevTables)) + waves * frozen_wave_bytes();
+}
+
+template <bool kDecode, int D>
+static hipError_t launch_frozen_one(const CodecArgs &a, uint32_t blocks, uint32_t wpb,
+                                    hipStream_t stream) {
+  const uint32_t lds = frozen_lds_bytes(wpb);
+  hipError_t e = hipFuncSetAttribute((const void *)frozen_kernel<kDecode, D>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((frozen_kernel<kDecode, D>), dim3(blocks), dim3(64 * wpb), lds, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_frozen(bool decode, const CodecArgs &a, uint32_t blocks, uint32_t wpb,
+                         hipStream_t stream) {
+  if (blocks == 0 || wpb == 0 || wpb > 8) return hipErrorInvalidValue;
+  if (!decode) return launch_frozen_one<false, 0>(a, blocks, wpb, stream);
+  switch (a.spec_bits) {
+    case 1: return launch_frozen_one<true, 1>(a, blocks, wpb, stream);
+    case 2: return launch_frozen_one<true, 2>(a, blocks, wpb, stream);
+    case 4: return launch_frozen_one<true, 4>(a, blocks, wpb, stream);
+    case 8: return launch_frozen_one<true, 8>(a, blocks, wpb, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_read_headers(const uint64_t *src, const uint64_t *len, uint32_t n, uint8_t *hdr,

@@ -80,12 +80,13 @@ struct Task {
   uint32_t aux;    // encoder: staging capacity; decoder: payload length
   uint16_t U;      // unit bits
   uint8_t tb;      // log2 slots per order table
-  uint8_t flags;   // kTaskTailPad | kTaskInitModel
+  uint8_t flags;   // kTaskTailPad | kTaskInitModel | kTaskFrozen
   uint32_t model;  // index of the initial model (kTaskInitModel)
   uint32_t pad;
 };
 constexpr uint8_t kTaskTailPad = 1;
 constexpr uint8_t kTaskInitModel = 2;
+constexpr uint8_t kTaskFrozen = 4;  // frozen model: run by frozen_kernel, skipped by the others
 
 // A trained initial model (prelearn) in device memory.  Order-0 states are
 // given by h0 (1..255) since order 0 has exactly 255 distinct slots.
@@ -153,6 +154,11 @@ uint32_t team_codec_lds_bytes(uint32_t Umax, uint32_t spec_bits, bool decode);
 bool team_supported(bool decode, uint32_t spec_bits);
 hipError_t launch_codec(bool decode, const CodecArgs &a, uint32_t Umax, uint32_t blocks,
                         uint32_t waves_per_block, bool team, hipStream_t stream);
+// The frozen-model codec (one wave per chunk; tasks without kTaskFrozen are
+// skipped): LDS per workgroup of `waves` waves, and the launcher.
+uint32_t frozen_lds_bytes(uint32_t waves);
+hipError_t launch_frozen(bool decode, const CodecArgs &a, uint32_t blocks,
+                         uint32_t waves_per_block, hipStream_t stream);
 hipError_t launch_read_headers(const uint64_t *src, const uint64_t *len, uint32_t n, uint8_t *hdr,
                                hipStream_t stream);
 hipError_t launch_build_decode_tasks(const ParseJob *jobs, uint32_t n, Task *tasks,

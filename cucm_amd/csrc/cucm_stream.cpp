@@ -45,8 +45,8 @@ int cucm_container_head_len(const uint8_t *in, uint64_t in_len, uint64_t *head_l
   if (!in || !head_len) return CUCM_E_PARAM;
   if (in_len < kHead) return CUCM_E_CORRUPT;
   const uint32_t flag = in[20];
-  if (flag > 1) return CUCM_E_CORRUPT;
-  *head_len = kHead + (flag ? be32(in + 12) : be32(in + 16));
+  if (flag > (1u | kHeaderFrozen)) return CUCM_E_CORRUPT;
+  *head_len = kHead + ((flag & 1u) ? be32(in + 12) : be32(in + 16));
   return CUCM_OK;
 }
 
@@ -140,7 +140,7 @@ int cucm_stream_decompress_begin(cucm_ctx *ctx, const uint8_t *head, uint64_t he
   if (P) {
     HostModel m(hp.table_bits);
     const uint8_t *seg = head + kHead;
-    if (head[20])
+    if (head[20] & 1u)
       prelearn_decode(seg, need - kHead, P, m, prelearn_out);
     else {
       std::memcpy(prelearn_out, seg, P);
@@ -188,6 +188,7 @@ int cucm_stream_decompress_next(cucm_stream *s, const uint8_t *records, uint64_t
   rec.U = s->p.unit_bits;
   rec.tb = s->p.table_bits;
   rec.batch = s->p.batch_chunks;
+  rec.frozen = (s->p.flags & CUCM_FLAG_FROZEN) != 0;
   rec.ext_model = s->has_model ? &s->dm : nullptr;
   int32_t st = 0;
   const int rc = decompress_host(s->ctx, 1, &records, &len, &out, &out_cap, out_len, &st, s->spec,

@@ -40,6 +40,15 @@ extern "C" {
  * forms decode identically; only the padded form is byte-identical to the
  * reference encoder. */
 #define CUCM_FLAG_NO_TAIL_PAD 0x1u
+/* Encoder option, the paper's "frozen model" (the reference ships no code
+ * for it): every bit of every chunk is predicted from the initial model --
+ * zero, or trained on the first prelearn_size bytes -- which is never
+ * updated, and each chunk codes exactly its own bits (no tail padding).
+ * The per-bit model and coder are the normal path's.  The container marks
+ * it with bit 1 (0x2) of the header's flag byte (offset 20), which the
+ * decoder reads back; the reference's reader does not know that bit.  With
+ * no updates the model is read-only: one copy serves every chunk. */
+#define CUCM_FLAG_FROZEN 0x2u
 
 typedef struct cucm_params {
   uint32_t chunk_size;    /* S, bytes per chunk (reference -s, default 1 MiB) */
@@ -63,7 +72,8 @@ const char *cucm_strerror(int code);
 uint64_t cucm_compress_bound(uint64_t in_len, const cucm_params *p);
 
 /* Reads the 21-byte header (cucm_decompress_v2.cu:1189-1203,
- * main's 4-byte size read :1297-1301). */
+ * main's 4-byte size read :1297-1301); out->flags gets CUCM_FLAG_FROZEN
+ * when the container was coded with the frozen model. */
 int cucm_peek_header(const uint8_t *in, uint64_t in_len, uint64_t *orig_len, cucm_params *out);
 
 /* ---- one-shot, host buffers ------------------------------------------ */

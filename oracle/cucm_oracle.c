@@ -360,6 +360,55 @@ int orc_decode_chunk(const uint8_t *pay, uint64_t plen, uint64_t L, uint64_t S,
 }
 
 /* ------------------------------------------------------------------ */
+/* Frozen model: every bit predicted from the initial model, never     */
+/* updated.  The reference has no code for this mode; its per-bit      */
+/* model (eval_bit, cucm_compress_v2.cu:711-792) and coder (:841-868)  */
+/* are the normal path's, only the updates and tail padding are gone.  */
+/* ------------------------------------------------------------------ */
+
+int64_t orc_encode_chunk_frozen(const uint8_t *chunk, uint64_t L, const orc_model *m,
+                                uint8_t *out, uint64_t cap) {
+  if (!chunk || !m || !out || L == 0) return ORC_E_PARAM;
+  uint32_t x1 = 0, x2 = 0xffffffffu;
+  uint64_t n = 0;
+  uint8_t *sink = out;
+  bitrec r;
+  for (uint64_t b = 0; b < 8 * L; ++b) {
+    eval_bit(m, chunk, b, &r);
+    if (n + 4 >= cap) sink = NULL; /* count only once the buffer is full */
+    orc_coder_step(&x1, &x2, r.pr, bit_of(chunk, b), sink, &n);
+  }
+  if (n + 1 > cap || sink == NULL) return ORC_E_SMALL;
+  out[n++] = (uint8_t)(x1 >> 24);
+  return (int64_t)n;
+}
+
+int orc_decode_chunk_frozen(const uint8_t *pay, uint64_t plen, uint64_t L, const orc_model *m,
+                            uint8_t *out) {
+  if (!pay || !m || !out || L == 0) return ORC_E_PARAM;
+  memset(out, 0, L);
+  uint64_t ip = 0;
+  uint32_t x = 0;
+  for (int i = 0; i < 4; ++i) x = (x << 8) | (ip < plen ? pay[ip++] : 255u);
+  uint32_t x1 = 0, x2 = 0xffffffffu;
+  bitrec r;
+  for (uint64_t b = 0; b < 8 * L; ++b) {
+    eval_bit(m, out, b, &r);
+    const uint32_t range = x2 - x1, p = (uint32_t)r.pr;
+    const uint32_t xmid = x1 + (range >> 12) * p + (((range & 0xfffu) * p) >> 12);
+    const int y = x <= xmid;
+    if (y) x2 = xmid; else x1 = xmid + 1u;
+    while (((x1 ^ x2) & 0xff000000u) == 0) {
+      x1 <<= 8;
+      x2 = (x2 << 8) | 255u;
+      x = (x << 8) | (ip < plen ? pay[ip++] : 255u);
+    }
+    out[b >> 3] |= (uint8_t)(y << (7 - (b & 7)));
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* Batch blobs                                                         */
 /* ------------------------------------------------------------------ */
 
@@ -406,9 +455,9 @@ static int assemble_blob(uint8_t **pays, const uint64_t *sizes, uint64_t C,
   return ORC_OK;
 }
 
-int orc_encode_batch(const uint8_t *in, uint64_t N, uint64_t S, uint32_t U,
-                     const orc_model *init, int tail_pad, int threads,
-                     uint8_t *blob, uint64_t cap, uint64_t *blob_len) {
+static int encode_batch_impl(const uint8_t *in, uint64_t N, uint64_t S, uint32_t U,
+                             const orc_model *init, int tail_pad, int frozen, int threads,
+                             uint8_t *blob, uint64_t cap, uint64_t *blob_len) {
   if (!in || !init || !blob || !blob_len || S == 0 || U == 0 || N == 0) return ORC_E_PARAM;
   const uint64_t C = (N + S - 1) / S;
   uint8_t **pays = (uint8_t **)calloc(C, sizeof(uint8_t *));
@@ -423,9 +472,10 @@ int orc_encode_batch(const uint8_t *in, uint64_t N, uint64_t S, uint32_t U,
     const uint64_t L = (c == (int64_t)C - 1) ? N - (uint64_t)c * S : S;
     const uint64_t bcap = chunk_bound(S);
     pays[c] = (uint8_t *)malloc(bcap);
-    int64_t r = pays[c] ? orc_encode_chunk(in + (uint64_t)c * S, L, S, U, init,
-                                           tail_pad && (c == (int64_t)C - 1), pays[c], bcap)
-                        : ORC_E_NOMEM;
+    int64_t r = !pays[c] ? ORC_E_NOMEM
+                : frozen ? orc_encode_chunk_frozen(in + (uint64_t)c * S, L, init, pays[c], bcap)
+                         : orc_encode_chunk(in + (uint64_t)c * S, L, S, U, init,
+                                            tail_pad && (c == (int64_t)C - 1), pays[c], bcap);
     if (r < 0) {
 #ifdef _OPENMP
 #pragma omp critical(orc_err)
@@ -438,6 +488,17 @@ int orc_encode_batch(const uint8_t *in, uint64_t N, uint64_t S, uint32_t U,
   for (uint64_t c = 0; c < C; ++c) free(pays[c]);
   free(pays); free(sizes);
   return err;
+}
+
+int orc_encode_batch(const uint8_t *in, uint64_t N, uint64_t S, uint32_t U,
+                     const orc_model *init, int tail_pad, int threads,
+                     uint8_t *blob, uint64_t cap, uint64_t *blob_len) {
+  return encode_batch_impl(in, N, S, U, init, tail_pad, 0, threads, blob, cap, blob_len);
+}
+
+int orc_encode_batch_frozen(const uint8_t *in, uint64_t N, uint64_t S, const orc_model *init,
+                            int threads, uint8_t *blob, uint64_t cap, uint64_t *blob_len) {
+  return encode_batch_impl(in, N, S, 32, init, 0, 1, threads, blob, cap, blob_len);
 }
 
 /* Literal restatement of compress_batch (cucm_compress_v2.cu:990-1115):
@@ -572,8 +633,9 @@ static int parse_blob(const uint8_t *blob, uint64_t blen, uint64_t C, uint64_t *
   return ORC_OK;
 }
 
-int orc_decode_batch(const uint8_t *blob, uint64_t blen, uint64_t N, uint64_t S,
-                     uint32_t U, const orc_model *init, int threads, uint8_t *out) {
+static int decode_batch_impl(const uint8_t *blob, uint64_t blen, uint64_t N, uint64_t S,
+                             uint32_t U, const orc_model *init, int frozen, int threads,
+                             uint8_t *out) {
   if (!blob || !init || !out || S == 0 || U == 0 || N == 0) return ORC_E_PARAM;
   const uint64_t C = (N + S - 1) / S;
   uint64_t *st = (uint64_t *)malloc((C + 1) * sizeof(uint64_t));
@@ -587,8 +649,10 @@ int orc_decode_batch(const uint8_t *blob, uint64_t blen, uint64_t N, uint64_t S,
 #endif
   for (int64_t c = 0; c < (int64_t)C; ++c) {
     const uint64_t L = (c == (int64_t)C - 1) ? N - (uint64_t)c * S : S;
-    int r = orc_decode_chunk(payload + st[c], st[c + 1] - st[c], L, S, U, init,
-                             out + (uint64_t)c * S);
+    int r = frozen ? orc_decode_chunk_frozen(payload + st[c], st[c + 1] - st[c], L, init,
+                                             out + (uint64_t)c * S)
+                   : orc_decode_chunk(payload + st[c], st[c + 1] - st[c], L, S, U, init,
+                                      out + (uint64_t)c * S);
     if (r) {
 #ifdef _OPENMP
 #pragma omp critical(orc_err)
@@ -599,6 +663,16 @@ int orc_decode_batch(const uint8_t *blob, uint64_t blen, uint64_t N, uint64_t S,
   (void)threads;
   free(st);
   return err;
+}
+
+int orc_decode_batch(const uint8_t *blob, uint64_t blen, uint64_t N, uint64_t S,
+                     uint32_t U, const orc_model *init, int threads, uint8_t *out) {
+  return decode_batch_impl(blob, blen, N, S, U, init, 0, threads, out);
+}
+
+int orc_decode_batch_frozen(const uint8_t *blob, uint64_t blen, uint64_t N, uint64_t S,
+                            const orc_model *init, int threads, uint8_t *out) {
+  return decode_batch_impl(blob, blen, N, S, 32, init, 1, threads, out);
 }
 
 /* Literal restatement of decompressbatch (cucm_decompress_v2.cu:898-1054)
@@ -842,9 +916,9 @@ uint64_t orc_compress_bound(uint64_t N, uint32_t S, uint32_t P, uint32_t batch_n
   return ORC_HEADER + (uint64_t)P + 4 * batches + 4 * C + C * chunk_bound(S);
 }
 
-int orc_compress(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
-                 uint32_t batch_num, uint32_t P, int tb, int tail_pad,
-                 int threads, uint8_t *out, uint64_t cap, uint64_t *out_len) {
+static int compress_impl(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
+                         uint32_t batch_num, uint32_t P, int tb, int tail_pad, int frozen,
+                         int threads, uint8_t *out, uint64_t cap, uint64_t *out_len) {
   if (!out || !out_len || (N && !in)) return ORC_E_PARAM;
   if (N > 0xffffffffull || S == 0 || U == 0 || U % 32 || U > 65504 || batch_num == 0 ||
       batch_num > 65535 || P > N || tb < 8 || tb > 25)
@@ -873,7 +947,7 @@ int orc_compress(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
   put_be32(out + 8, S);
   put_be32(out + 12, (uint32_t)Pc);
   put_be32(out + 16, P);
-  out[20] = (uint8_t)flag;
+  out[20] = (uint8_t)(flag | (frozen ? ORC_FLAG_FROZEN : 0));
   if (seg) memcpy(out + ORC_HEADER, flag ? pseg : in, seg);
   free(pseg);
   uint64_t pos = ORC_HEADER + seg;
@@ -886,8 +960,8 @@ int orc_compress(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
     const uint64_t nb = (body - off < bbytes) ? body - off : bbytes;
     if (pos + 4 > cap) { err = ORC_E_SMALL; break; }
     uint64_t blen = 0;
-    err = orc_encode_batch(in + P + off, nb, S, U, m, tail_pad, threads, out + pos + 4,
-                           cap - pos - 4, &blen);
+    err = encode_batch_impl(in + P + off, nb, S, U, m, tail_pad, frozen, threads,
+                            out + pos + 4, cap - pos - 4, &blen);
     if (err) break;
     put_le32(out + pos, (uint32_t)blen);
     pos += 4 + blen;
@@ -896,6 +970,18 @@ int orc_compress(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
   if (err) return err;
   *out_len = pos;
   return ORC_OK;
+}
+
+int orc_compress(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
+                 uint32_t batch_num, uint32_t P, int tb, int tail_pad,
+                 int threads, uint8_t *out, uint64_t cap, uint64_t *out_len) {
+  return compress_impl(in, N, S, U, batch_num, P, tb, tail_pad, 0, threads, out, cap, out_len);
+}
+
+int orc_compress_frozen(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
+                        uint32_t batch_num, uint32_t P, int tb, int threads, uint8_t *out,
+                        uint64_t cap, uint64_t *out_len) {
+  return compress_impl(in, N, S, U, batch_num, P, tb, 0, 1, threads, out, cap, out_len);
 }
 
 int orc_decompress(const uint8_t *in, uint64_t len, int threads, uint8_t *out,
@@ -907,11 +993,11 @@ int orc_decompress(const uint8_t *in, uint64_t len, int threads, uint8_t *out,
   const uint64_t S = get_be32(in + 8);
   const uint64_t Pc = get_be32(in + 12);
   const uint64_t P = get_be32(in + 16);
-  const int flag = in[20];
+  const int flag = in[20] & 1, frozen = (in[20] & ORC_FLAG_FROZEN) != 0;
   const uint32_t U = ufield & ~31u;
   const int tb = (ufield & 31u) ? (int)(ufield & 31u) : 25;
   if (N > cap) return ORC_E_SMALL;
-  if (P > N || flag > 1 || (N > P && (S == 0 || U == 0 || batch_num == 0)) || tb < 8)
+  if (P > N || in[20] > (1 | ORC_FLAG_FROZEN) || (N > P && (S == 0 || U == 0 || batch_num == 0)) || tb < 8)
     return ORC_E_CORRUPT;
   const uint64_t seg = flag ? Pc : P;
   if (ORC_HEADER + seg > len) return ORC_E_CORRUPT;
@@ -936,7 +1022,7 @@ int orc_decompress(const uint8_t *in, uint64_t len, int threads, uint8_t *out,
     if (pos + 4 > len) { err = ORC_E_CORRUPT; break; }
     const uint64_t blen = get_le32(in + pos);
     if (pos + 4 + blen > len) { err = ORC_E_CORRUPT; break; }
-    err = orc_decode_batch(in + pos + 4, blen, nb, S, U, m, threads, out + P + off);
+    err = decode_batch_impl(in + pos + 4, blen, nb, S, U, m, frozen, threads, out + P + off);
     if (err) break;
     pos += 4 + blen;
   }

@@ -93,6 +93,26 @@ int orc_decode_chunk(const uint8_t *pay, uint64_t plen, uint64_t L,
                      uint64_t S, uint32_t U, const orc_model *init,
                      uint8_t *out);
 
+/* ---- frozen model --------------------------------------------------- */
+/* The reference has no code for this mode (the paper's "frozen model":
+ * no runtime updates after prelearn).  This build's definition: every bit
+ * of the chunk is predicted from `m` exactly as the normal path predicts
+ * from its model (slots, statep, stretch, mixer context, mix, pr bump), the
+ * model is never updated, and exactly 8L bits are coded (no tail padding),
+ * then the one flush byte.  The container marks it with bit 1 of the flag
+ * byte (offset 20). */
+#define ORC_FLAG_FROZEN 2
+int64_t orc_encode_chunk_frozen(const uint8_t *chunk, uint64_t L,
+                                const orc_model *m, uint8_t *out,
+                                uint64_t cap);
+int orc_decode_chunk_frozen(const uint8_t *pay, uint64_t plen, uint64_t L,
+                            const orc_model *m, uint8_t *out);
+int orc_encode_batch_frozen(const uint8_t *in, uint64_t N, uint64_t S,
+                            const orc_model *init, int threads, uint8_t *blob,
+                            uint64_t cap, uint64_t *blob_len);
+int orc_decode_batch_frozen(const uint8_t *blob, uint64_t blen, uint64_t N,
+                            uint64_t S, const orc_model *init, int threads,
+                            uint8_t *out);
 /* ---- batch blob: 4C big-endian end offsets + payloads ---------------- */
 uint64_t orc_blob_bound(uint64_t N, uint64_t S);
 int orc_encode_batch(const uint8_t *in, uint64_t N, uint64_t S, uint32_t U,
@@ -124,6 +144,11 @@ uint64_t orc_compress_bound(uint64_t N, uint32_t S, uint32_t P,
 int orc_compress(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
                  uint32_t batch_num, uint32_t P, int tb, int tail_pad,
                  int threads, uint8_t *out, uint64_t cap, uint64_t *out_len);
+/* orc_compress with the frozen model: the header's flag byte carries
+ * ORC_FLAG_FROZEN and orc_decompress decodes it accordingly. */
+int orc_compress_frozen(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
+                        uint32_t batch_num, uint32_t P, int tb, int threads,
+                        uint8_t *out, uint64_t cap, uint64_t *out_len);
 int orc_decompress(const uint8_t *in, uint64_t len, int threads, uint8_t *out,
                    uint64_t cap, uint64_t *out_len);
 

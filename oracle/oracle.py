@@ -67,6 +67,12 @@ def lib():
         L.orc_compress_bound.restype = u64
         L.orc_compress.argtypes = [u8p, u64, u32, u32, u32, u32, ci, ci, ci, vp, u64, vp]
         L.orc_decompress.argtypes = [u8p, u64, ci, vp, u64, vp]
+        L.orc_encode_chunk_frozen.argtypes = [u8p, u64, vp, vp, u64]
+        L.orc_encode_chunk_frozen.restype = i64
+        L.orc_decode_chunk_frozen.argtypes = [u8p, u64, u64, vp, vp]
+        L.orc_encode_batch_frozen.argtypes = [u8p, u64, u64, vp, ci, vp, u64, vp]
+        L.orc_decode_batch_frozen.argtypes = [u8p, u64, u64, u64, vp, ci, vp]
+        L.orc_compress_frozen.argtypes = [u8p, u64, u32, u32, u32, u32, ci, ci, vp, u64, vp]
         _lib = L
     return _lib
 
@@ -198,16 +204,56 @@ def decode_batch(blob: bytes, N: int, S: int, U: int, model: Model, threads=0, l
 
 
 def compress(data: bytes, S: int, U: int = 128, batch_num: int = 100, P: int = 0, tb: int = 25,
-             tail_pad=True, threads=0) -> bytes:
+             tail_pad=True, threads=0, frozen=False) -> bytes:
+    """Container of `data`; with frozen=True every chunk is coded with the
+    initial (prelearned or zero) model and no updates (orc_compress_frozen)."""
     L = lib()
     cap = L.orc_compress_bound(len(data), S, P, batch_num)
     if cap == 0:
         raise OracleError("bad params")
     out = ctypes.create_string_buffer(cap)
     n = ctypes.c_uint64(0)
-    _check(L.orc_compress(data, len(data), S, U, batch_num, P, tb, 1 if tail_pad else 0, threads,
-                          out, cap, ctypes.byref(n)), "compress")
+    if frozen:
+        rc = L.orc_compress_frozen(data, len(data), S, U, batch_num, P, tb, threads, out, cap,
+                                   ctypes.byref(n))
+    else:
+        rc = L.orc_compress(data, len(data), S, U, batch_num, P, tb, 1 if tail_pad else 0,
+                            threads, out, cap, ctypes.byref(n))
+    _check(rc, "compress")
     return out.raw[: n.value]
+
+
+def encode_chunk_frozen(chunk: bytes, model: Model) -> bytes:
+    cap = 12 * len(chunk) + 16
+    out = ctypes.create_string_buffer(cap)
+    n = lib().orc_encode_chunk_frozen(chunk, len(chunk), model.ptr, out, cap)
+    if n < 0:
+        raise OracleError("encode_chunk_frozen %d" % n)
+    return out.raw[:n]
+
+
+def decode_chunk_frozen(payload: bytes, L: int, model: Model) -> bytes:
+    out = ctypes.create_string_buffer(max(L, 1))
+    _check(lib().orc_decode_chunk_frozen(payload, len(payload), L, model.ptr, out),
+           "decode_chunk_frozen")
+    return out.raw[:L]
+
+
+def encode_batch_frozen(data: bytes, S: int, model: Model, threads=0) -> bytes:
+    L = lib()
+    cap = L.orc_blob_bound(len(data), S)
+    out = ctypes.create_string_buffer(cap)
+    n = ctypes.c_uint64(0)
+    _check(L.orc_encode_batch_frozen(data, len(data), S, model.ptr, threads, out, cap,
+                                     ctypes.byref(n)), "encode_batch_frozen")
+    return out.raw[: n.value]
+
+
+def decode_batch_frozen(blob: bytes, N: int, S: int, model: Model, threads=0) -> bytes:
+    out = ctypes.create_string_buffer(max(N, 1))
+    _check(lib().orc_decode_batch_frozen(blob, len(blob), N, S, model.ptr, threads, out),
+           "decode_batch_frozen")
+    return out.raw[:N]
 
 
 def decompress(blob: bytes, threads=0) -> bytes:

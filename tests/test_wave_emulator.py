@@ -135,3 +135,43 @@ def test_emulated_team_encoder_matches_oracle(emu, oracle, tmp_path, kind, N, S,
             ref = oracle.encode_chunk(data[c * S:(c + 1) * S], S, U, oracle.Model(tb),
                                       tail_pad=c == C - 1)
             assert pays[c] == ref, "seed %d chunk %d" % (seed, c)
+
+
+FROZEN_CASES = [
+    # (kind, N, S, tb, prelearn bytes, spec_bits)
+    ("markov2", 6000, 2048, 12, 3000, 4),
+    ("mixed", 5000, 1000, 14, 0, 1),
+    ("structured", 4000, 4096, 11, 2500, 8),
+    ("random", 1500, 512, 9, 700, 2),
+    ("markov2", 33, 4096, 16, 400, 4),
+    ("markov2", 700, 64, 10, 0, 8),
+]
+
+
+@pytest.mark.parametrize("kind,N,S,tb,P,spec", FROZEN_CASES)
+def test_emulated_frozen_codec_matches_oracle(emu, oracle, tmp_path, kind, N, S, tb, P, spec):
+    """The frozen-model encoder and decoder (cucm_frozen.h) on one emulated
+    wave that keeps its model loaded across chunks: every chunk payload equal
+    to the oracle's frozen payload, every chunk decoded losslessly."""
+    data = _data(kind, N)
+    pre = datagen.markov2(P, 21).tobytes() if P else b""
+    inp, enc, dec, pf = tmp_path / "in", tmp_path / "enc", tmp_path / "dec", tmp_path / "pre"
+    inp.write_bytes(data)
+    pf.write_bytes(pre)
+    model = oracle.Model(tb)
+    if P:
+        oracle.prelearn_train(pre, model)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    pre_arg = str(pf) if P else "-"
+    r = subprocess.run([emu, "fenc", str(inp), str(S), str(tb), pre_arg, str(enc)],
+                       capture_output=True, text=True, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
+    pays = _payloads(enc.read_bytes())
+    C = (N + S - 1) // S
+    assert len(pays) == C
+    for c in range(C):
+        assert pays[c] == oracle.encode_chunk_frozen(data[c * S:(c + 1) * S], model), "chunk %d" % c
+    r = subprocess.run([emu, "fdec", str(enc), str(N), str(S), str(tb), str(spec), pre_arg, str(dec)],
+                       capture_output=True, text=True, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert dec.read_bytes() == data

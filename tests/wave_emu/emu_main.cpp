@@ -13,9 +13,15 @@
 //       the same through the team decoder (cucm_team.h): every chunk in turn
 //       on one workgroup of kTeamWaves waves and one LDS, waves stepped in
 //       an order drawn from seed
+//   wave_emu fenc <input> <S> <tb> <prelearn|-> <output>
+//       every chunk through the frozen-model encoder (cucm_frozen.h) on one
+//       wave, with the zero model or the model trained on the prelearn file
+//   wave_emu fdec <payloads> <N> <S> <tb> <spec_bits> <prelearn|-> <output>
+//       the same through the frozen-model decoder
 #define CUCM_WAVE_EMULATION 1
 #include "wave_emulation.h"
 // (the device code follows the emulation shims)
+#include "cucm_frozen.h"
 #include "cucm_team.h"
 #include "cucm_host_model.h"
 
@@ -262,6 +268,142 @@ static void decode_team_d(int spec, const std::vector<uint8_t> &in, uint8_t *out
   }
 }
 
+// The frozen codec's model: zero (no file) or trained on the prelearn file.
+struct FrozenSetup {
+  HostModel m;
+  DevInitModel d{};
+  bool init = false;
+  FrozenSetup(uint32_t tb, const char *prelearn) : m((int)tb) {
+    if (std::string(prelearn) != "-") {
+      const std::vector<uint8_t> pre = read_file(prelearn);
+      prelearn_train(pre.data(), pre.size(), m);
+      d.states = m.states.data() + ((size_t)1 << tb);  // orders 1..7
+      std::memcpy(d.statep, m.statep, sizeof d.statep);
+      std::memcpy(d.w, m.w, sizeof d.w);
+      order0_states(m, d.st0);
+      d.tb = tb;
+      init = true;
+    }
+  }
+};
+
+// Every chunk through one frozen wave (its model stays loaded across
+// chunks, as in the persistent kernel).
+static std::vector<uint8_t> frozen_encode_all(const std::vector<uint8_t> &in, uint32_t S,
+                                              uint32_t tb, const FrozenSetup &fs) {
+  const uint64_t N = in.size();
+  const uint64_t C = (N + S - 1) / S;
+  std::vector<Task> tasks(C);
+  std::vector<uint8_t *> srcs(C), dsts(C);
+  const uint32_t cap = align16(S + S / 2 + 256);
+  for (uint64_t c = 0; c < C; ++c) {
+    const uint32_t L = (uint32_t)std::min<uint64_t>(S, N - c * S);
+    srcs[c] = (uint8_t *)malloc(L);
+    memcpy(srcs[c], in.data() + c * S, L);
+    dsts[c] = (uint8_t *)malloc(cap);
+    Task &tk = tasks[c];
+    tk = Task{};
+    tk.src = (uint64_t)(uintptr_t)srcs[c];
+    tk.dst = (uint64_t)(uintptr_t)dsts[c];
+    tk.L = L;
+    tk.S = S;
+    tk.aux = cap;
+    tk.U = 32;
+    tk.tb = (uint8_t)tb;
+    tk.flags = kTaskFrozen | (fs.init ? kTaskInitModel : 0);
+  }
+  const size_t tab = align16(sizeof(DevTables));
+  const size_t bytes = tab + frozen_wave_bytes();
+  uint8_t *base = (uint8_t *)aligned_alloc(16, bytes);
+  memset(base, 0xa5, bytes);
+  memcpy(base, &host_tables(), sizeof(DevTables));
+  const DevTables *T = (const DevTables *)base;
+  std::vector<uint64_t> result(C, 0);
+  CodecArgs a{};
+  a.tasks = tasks.data();
+  a.ntasks = (uint32_t)C;
+  a.result = result.data();
+  a.init = &fs.d;
+  emu::run_wave([&](uint32_t lane) {
+    const FrozenMem F = carve_frozen(base + tab);
+    uint32_t loaded = kNoModel;
+    for (uint64_t c = 0; c < C; ++c) {
+      const GLB uint8_t *states = frozen_model(a, tasks[c], F, T, loaded, lane);
+      frozen_encode_task(a, tasks[c], (uint32_t)c, F, T, states, lane);
+    }
+  });
+  std::vector<uint8_t> out;
+  for (uint64_t c = 0; c < C; ++c) {
+    if (result[c] & kResultOverflow) {
+      fprintf(stderr, "staging overflow\n");
+      exit(3);
+    }
+    const uint32_t n = (uint32_t)result[c];
+    for (int i = 0; i < 4; ++i) out.push_back((uint8_t)(n >> (8 * i)));
+    out.insert(out.end(), dsts[c], dsts[c] + n);
+    free(srcs[c]);
+    free(dsts[c]);
+  }
+  free(base);
+  return out;
+}
+
+template <int D>
+static void frozen_decode_all(const std::vector<uint8_t> &in, uint8_t *out, uint64_t N,
+                              uint32_t S, uint32_t tb, const FrozenSetup &fs) {
+  const uint64_t C = (N + S - 1) / S;
+  std::vector<Task> tasks(C);
+  std::vector<uint8_t *> srcs(C), dsts(C);
+  size_t pos = 0;
+  for (uint64_t c = 0; c < C; ++c) {
+    const uint32_t L = (uint32_t)std::min<uint64_t>(S, N - c * S);
+    uint32_t n = 0;
+    for (int i = 0; i < 4; ++i) n |= (uint32_t)in[pos + i] << (8 * i);
+    pos += 4;
+    srcs[c] = (uint8_t *)malloc(n ? n : 1);
+    memcpy(srcs[c], in.data() + pos, n);
+    pos += n;
+    dsts[c] = (uint8_t *)malloc(L);
+    Task &tk = tasks[c];
+    tk = Task{};
+    tk.src = (uint64_t)(uintptr_t)srcs[c];
+    tk.dst = (uint64_t)(uintptr_t)dsts[c];
+    tk.L = L;
+    tk.S = S;
+    tk.aux = n;
+    tk.U = 32;
+    tk.tb = (uint8_t)tb;
+    tk.flags = kTaskFrozen | (fs.init ? kTaskInitModel : 0);
+  }
+  const size_t tab = align16(sizeof(DevTables));
+  const size_t bytes = tab + frozen_wave_bytes();
+  uint8_t *base = (uint8_t *)aligned_alloc(16, bytes);
+  memset(base, 0xa5, bytes);
+  memcpy(base, &host_tables(), sizeof(DevTables));
+  const DevTables *T = (const DevTables *)base;
+  std::vector<uint64_t> result(C, 0);
+  CodecArgs a{};
+  a.tasks = tasks.data();
+  a.ntasks = (uint32_t)C;
+  a.spec_bits = D;
+  a.result = result.data();
+  a.init = &fs.d;
+  emu::run_wave([&](uint32_t lane) {
+    const FrozenMem F = carve_frozen(base + tab);
+    uint32_t loaded = kNoModel;
+    for (uint64_t c = 0; c < C; ++c) {
+      const GLB uint8_t *states = frozen_model(a, tasks[c], F, T, loaded, lane);
+      frozen_decode_task<D>(a, tasks[c], (uint32_t)c, F, T, states, lane);
+    }
+  });
+  for (uint64_t c = 0; c < C; ++c) {
+    memcpy(out + c * S, dsts[c], tasks[c].L);
+    free(srcs[c]);
+    free(dsts[c]);
+  }
+  free(base);
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) return 2;
   const std::string mode = argv[1];
@@ -348,6 +490,30 @@ int main(int argc, char **argv) {
     write_file(argv[9], out);
     return 0;
   }
-  fprintf(stderr, "usage: wave_emu enc|dec|tenc|tdec ...\n");
+  if (mode == "fenc" && argc == 7) {
+    const std::vector<uint8_t> in = read_file(argv[2]);
+    const uint32_t S = atoi(argv[3]), tb = atoi(argv[4]);
+    const FrozenSetup fs(tb, argv[5]);
+    write_file(argv[6], frozen_encode_all(in, S, tb, fs));
+    return 0;
+  }
+  if (mode == "fdec" && argc == 9) {
+    const std::vector<uint8_t> in = read_file(argv[2]);
+    const uint64_t N = strtoull(argv[3], nullptr, 10);
+    const uint32_t S = atoi(argv[4]), tb = atoi(argv[5]);
+    const int spec = atoi(argv[6]);
+    const FrozenSetup fs(tb, argv[7]);
+    std::vector<uint8_t> out(N);
+    switch (spec) {
+      case 1: frozen_decode_all<1>(in, out.data(), N, S, tb, fs); break;
+      case 2: frozen_decode_all<2>(in, out.data(), N, S, tb, fs); break;
+      case 4: frozen_decode_all<4>(in, out.data(), N, S, tb, fs); break;
+      case 8: frozen_decode_all<8>(in, out.data(), N, S, tb, fs); break;
+      default: fprintf(stderr, "spec bits\n"); return 2;
+    }
+    write_file(argv[8], out);
+    return 0;
+  }
+  fprintf(stderr, "usage: wave_emu enc|dec|tenc|tdec|fenc|fdec ...\n");
   return 2;
 }
