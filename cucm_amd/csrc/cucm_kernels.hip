@@ -132,14 +132,14 @@ __global__ void __launch_bounds__(512) frozen_kernel(CodecArgs a) {
 // (cucm_compress_v2.cu:870-914, 1242-1244).
 // ---------------------------------------------------------------------------
 
-__global__ void layout_kernel(const AsmJob *jobs, uint32_t njobs, const uint8_t *headers,
-                              const uint64_t *result, uint64_t *task_dst, uint64_t *job_len) {
+__global__ void layout_kernel(const AsmJob *jobs, uint32_t njobs, const uint64_t *result,
+                              uint64_t *task_dst, uint64_t *job_len) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= njobs) return;
   const AsmJob J = jobs[j];
   uint8_t *o = (uint8_t *)(uintptr_t)J.out_base;
   const uint32_t lenbytes = J.blob_only ? 0u : 4u;
-  for (uint32_t i = 0; i < J.header_len; ++i) o[i] = headers[J.header_off + i];
+  // (the header bytes are copied by copy_headers_kernel)
   uint64_t pos = J.header_len;
   for (uint32_t c0 = 0; c0 < J.ntasks; c0 += J.batch_chunks) {
     const uint32_t C = min(J.batch_chunks, J.ntasks - c0);
@@ -166,6 +166,20 @@ __global__ void layout_kernel(const AsmJob *jobs, uint32_t njobs, const uint8_t 
     pos = bstart + lenbytes + e;
   }
   job_len[j] = pos;
+}
+
+// One wave per job: its header and prelearn segment (up to 21 + P bytes)
+// into the container, the lanes striding over it.
+__global__ void copy_headers_kernel(const AsmJob *jobs, uint32_t njobs, const uint8_t *headers) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t j = w0; j < njobs; j += nw) {
+    const AsmJob J = jobs[j];
+    uint8_t *o = (uint8_t *)(uintptr_t)J.out_base;
+    const uint8_t *h = headers + J.header_off;
+    for (uint64_t i = lane; i < J.header_len; i += 64) o[i] = h[i];
+  }
 }
 
 // One wave per task: copy the staged payload to its container position.
@@ -414,7 +428,10 @@ hipError_t launch_assemble(const AsmJob *jobs, uint32_t njobs, const uint8_t *he
                            uint64_t *task_dst, uint64_t *job_len, hipStream_t stream) {
   if (njobs) {
     hipLaunchKernelGGL(layout_kernel, dim3((njobs + 63) / 64), dim3(64), 0, stream, jobs, njobs,
-                       headers, result, task_dst, job_len);
+                       result, task_dst, job_len);
+    const uint32_t hblocks = min((njobs + 3) / 4, 4096u);
+    hipLaunchKernelGGL(copy_headers_kernel, dim3(hblocks), dim3(256), 0, stream, jobs, njobs,
+                       headers);
   }
   if (ntasks) {
     const uint32_t blocks = min((ntasks + 3) / 4, 4096u);

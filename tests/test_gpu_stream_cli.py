@@ -101,3 +101,17 @@ def test_cli_chunknum_truncates_the_input(ctx, tmp_path):
     src.write_bytes(data)
     _tool([COMP, "-f", str(src), "-o", str(enc), "-s", "8K", "-c", "5", "-t", "16", "-u", "64"])
     assert enc.read_bytes() == ctx.compress(data[:5 * 8192], make_params(8192, 64, 100, 0, 16))
+
+
+def test_cli_frozen_round_trip(ctx, oracle, tmp_path):
+    """--frozen: the container equals the oracle's frozen container, and the
+    decompressor reads the mode from the header."""
+    data = datagen.markov2(150_000, 79).tobytes()
+    src, enc, dec = tmp_path / "in", tmp_path / "in.cucm", tmp_path / "out"
+    src.write_bytes(data)
+    _tool([COMP, "-f", str(src), "-o", str(enc), "-s", "16K", "-b", "3", "-t", "16",
+           "-p", "20000", "--frozen"])
+    blob = enc.read_bytes()
+    assert blob == oracle.compress(data, 16384, 128, 3, 20000, 16, frozen=True)
+    _tool([DECOMP, "-i", str(enc), "-o", str(dec), "-a", "3"])
+    assert dec.read_bytes() == data
