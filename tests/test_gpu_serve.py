@@ -1,6 +1,6 @@
 """Serving loop (cucm_serve_start / cucm_submit / cucm_poll): requests of
-mixed kinds, sizes and parameters submitted together are batched into shared
-launches on concurrent lanes, and every result equals the same request run
+mixed kinds, sizes and parameters (frozen-model ones among them) submitted
+together are batched into shared launches on concurrent lanes, and every result equals the same request run
 alone -- bit-exact, since the codec's output depends only on the request."""
 import numpy as np
 import pytest
@@ -14,18 +14,20 @@ pytestmark = pytest.mark.gpu
 def _requests():
     rng = np.random.default_rng(23)
     reqs = []
-    for i in range(36):
+    for i in range(40):
         n = int(rng.choice([0, 1, 7, 300, 5000, 40000, 150000, 400000]))
         data = datagen.markov2(n, 300 + i).tobytes() if n else b""
-        kind = i % 4
+        kind = i % 5
         if kind == 0:
             p = make_params(16384, 128, 100, 0, 18)
         elif kind == 1:
             p = make_params(32768, 64, 3, 0, 16)
         elif kind == 2:
             p = make_params(8192, 256, 100, min(n, 2000), 14)
-        else:
+        elif kind == 3:
             p = make_params(65536, 96, 7, 0, 20, tail_pad=False)
+        else:  # frozen model: runs in the same launches as the others
+            p = make_params(16384, 128, 4, min(n, 3000), 16, frozen=True)
         reqs.append((data, p))
     return reqs
 
