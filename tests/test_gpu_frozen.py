@@ -102,3 +102,26 @@ def test_frozen_stream_equals_one_shot(fctx, oracle):
     assert same(fctx.decompress_stream(one, 4, records_per_call=2), data)
     orig, hp = cucm_amd.peek_header(one)
     assert orig == len(data) and hp.flags & 2
+
+
+def test_expanding_chunks_rerun_with_the_strict_bound(fctx, oracle):
+    """Chunks whose payload outgrows the 1.5 S + 256 staging region go
+    through the second launch into the strict-bound region: a model
+    prelearned on zeros pays ~4 bytes per 0x1f byte (its mixer trusts states
+    that predict 0).  Expanding and ordinary chunks, and a normal job, in one
+    call; containers equal the oracle's and decode losslessly."""
+    S, P = 4096, 5000
+    body = (datagen.markov2(3 * S, 3).tobytes() + b"\x1f" * (2 * S)
+            + datagen.markov2(S + 100, 4).tobytes())
+    data = bytes(P) + body
+    model = oracle.Model(14)
+    oracle.prelearn_train(bytes(P), model)
+    assert len(oracle.encode_chunk_frozen(body[3 * S:4 * S], model)) > S + S // 2 + 256
+    blob = oracle.compress(data, S, 128, 2, P, 14, frozen=True)
+    other = datagen.mixed(30000, 6).tobytes()
+    oblob = oracle.compress(other, S, 128, 100, 0, 14)
+    got = fctx.compress_many([data, other], [make_params(S, 128, 2, P, 14, frozen=True),
+                                             make_params(S, 128, 100, 0, 14)])
+    assert same(got, [blob, oblob])
+    for spec in (1, 4, 8):
+        assert same(fctx.decompress(blob, spec), data)
