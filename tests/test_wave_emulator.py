@@ -50,6 +50,12 @@ CASES = [
 
 
 def _data(kind, n):
+    if kind == "zeros":
+        return bytes(n)
+    if kind == "runs":  # runs of 1..39 equal bytes: a slot hit again by the next byte
+        rng = np.random.default_rng(4)
+        return b"".join(bytes([int(rng.integers(0, 256))]) * int(rng.integers(1, 40))
+                        for _ in range(n // 4 + 1))[:n]
     if kind == "markov2":
         return datagen.markov2(n, 3).tobytes()
     if kind == "mixed":
@@ -114,7 +120,18 @@ def test_emulated_team_decoder_is_lossless(emu, oracle, tmp_path, kind, N, S, U,
         assert dec.read_bytes() == data, "seed %d" % seed
 
 
-@pytest.mark.parametrize("kind,N,S,U,tb,spec,seeds", TEAM_CASES)
+# 2^8- and 2^9-slot tables make two bits of a byte, or bits of consecutive
+# bytes, share slots often (the resolve's slow paths); zeros and runs make
+# the same bit hit the same slot byte after byte (its forwarded case).
+TEAM_ENC_CASES = TEAM_CASES + [
+    ("zeros", 3000, 1024, 128, 8, 4, (1, 2)),
+    ("random", 3000, 1024, 128, 8, 4, (3,)),
+    ("markov2", 8000, 4096, 256, 8, 4, (4,)),
+    ("runs", 6000, 2048, 128, 10, 4, (5, 6)),
+]
+
+
+@pytest.mark.parametrize("kind,N,S,U,tb,spec,seeds", TEAM_ENC_CASES)
 def test_emulated_team_encoder_matches_oracle(emu, oracle, tmp_path, kind, N, S, U, tb, spec,
                                               seeds):
     """The team encoder (cucm_team.h): every chunk in turn on one emulated
