@@ -3,8 +3,8 @@
  *
  * PRNG: splitmix64 seeding xoshiro256** (public-domain algorithms of
  * Vigna/Blackman).  Every generator is a pure function of (n, seed) and is
- * OpenMP-parallel over independent regions, so the GPU box regenerates the
- * same bytes quickly.
+ * OpenMP-parallel over independent regions, so any host regenerates the same
+ * bytes quickly.
  *
  *   markov2     order-2 Markov source over 64 symbols (bytes 0x20+s); each of
  *               the 4096 (a, b) rows is Zipf(s) over ranks 1..64 mapped
