@@ -190,6 +190,7 @@ int validate_params(const cucm_params &p, uint64_t n) {
   if (p.chunk_size == 0 || p.unit_bits == 0 || p.unit_bits % 32 || p.batch_chunks == 0)
     return CUCM_E_PARAM;
   if (p.table_bits < 8 || p.table_bits > 25) return CUCM_E_PARAM;
+  if (p.flags & ~(CUCM_FLAG_NO_TAIL_PAD | CUCM_FLAG_FROZEN)) return CUCM_E_PARAM;
   if (p.prelearn_size > n) return CUCM_E_PARAM;
   if (p.unit_bits > kMaxUnitBits) return CUCM_E_UNSUPPORTED;
   return CUCM_OK;

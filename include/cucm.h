@@ -60,7 +60,7 @@ typedef struct cucm_params {
   uint8_t spec_bits;      /* decoder bits per speculative round: 1, 2, 4 or 8
                              (0 = default 4); never changes the stream */
   uint16_t reserved;
-  uint32_t flags;         /* CUCM_FLAG_* */
+  uint32_t flags;         /* CUCM_FLAG_*; other bits are rejected (CUCM_E_PARAM) */
 } cucm_params;
 
 /* Reference defaults: S = 1 MiB, U = 128, batch 100, P = 0, table_bits 25. */

@@ -149,6 +149,11 @@ def test_bad_params_rejected(ctx):
     with pytest.raises(cucm_amd.CucmError) as e:
         ctx.compress(b"abc", make_params(4096, 512))
     assert e.value.code == -7
+    p = make_params(4096, 128)
+    p.flags = 0x4  # no such option
+    with pytest.raises(cucm_amd.CucmError) as e:
+        ctx.compress(b"abc", p)
+    assert e.value.code == -1
 
 
 def test_one_shot_api(oracle):
