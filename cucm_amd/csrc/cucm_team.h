@@ -766,10 +766,12 @@ CUCM_DEV void encode_predict_team(const WaveMem &M, const LDS DevTables *T, uint
 }
 
 // P3 of wave 1: the state each StateMap update of the unit sees (update_states'
-// resolution for nbytes bytes; lane (k, j) = (order, bit of the byte)).  The bytes with an in-byte collision
-// are found first; each byte's slot, unit-start state and bit are loaded a
-// byte ahead, so a byte costs the dependent state lookup, nex lookup and
-// store alone.
+// resolution for nbytes <= NB bytes; lane (k, j) = (order, bit of the
+// byte)).  The bytes with an in-byte collision are found first, then every
+// byte's slot, unit-start state and bit are loaded into registers (one
+// packed word per byte, all loads in flight together), so the fully unrolled
+// byte loop carries only the dependent state lookup, nex lookup and stores.
+template <uint32_t NB>
 CUCM_DEV void team_resolve_unit(const TeamMem &X, const LDS DevTables *T, uint32_t nbytes,
                                 uint32_t Umax, uint32_t lane) {
   const WaveMem &M = X.M;
@@ -802,25 +804,42 @@ CUCM_DEV void team_resolve_unit(const TeamMem &X, const LDS DevTables *T, uint32
   LDS uint16_t *tv = k ? M.tval + (k - 1) * TS : X.t0;
   LDS uint8_t *rgk = M.rg + k * Umax;
   const LDS uint16_t *rik = M.ridx + k * Umax;
-  uint32_t i = rik[j], g = rgk[j], y = ((uint32_t)M.ub[8] >> (7 - j)) & 1u;
-  for (uint32_t b = 0; b < nbytes; ++b) {
-    const uint32_t r = 8 * b + j;
-    const uint32_t bn = min(b + 1, nbytes - 1);
-    const uint32_t i_n = rik[8 * bn + j], g_n = rgk[8 * bn + j];
-    const uint32_t y_n = ((uint32_t)M.ub[8 + bn] >> (7 - j)) & 1u;
-    const uint32_t reps = ((cmask >> b) & 1u) ? 8u : 1u;
-    for (uint32_t q = 0; q < reps; ++q) {
-      if (reps == 1u || q == j) {
+  // byte b of the unit: slot index | unit-start state << 16 | bit << 24
+  // (read before any of the unit's resolved states overwrite rg)
+  uint32_t in[NB];
+#pragma unroll
+  for (uint32_t b = 0; b < NB; ++b) {
+    const uint32_t bb = min(b, nbytes - 1);
+    in[b] = (uint32_t)rik[8 * bb + j] | ((uint32_t)rgk[8 * bb + j] << 16) |
+            ((((uint32_t)M.ub[8 + bb] >> (7 - j)) & 1u) << 24);
+  }
+  // all inputs in registers before the chain starts, so no wait inside it
+  // also waits for the previous byte's stores
+  wait_lgkm();
+#pragma unroll
+  for (uint32_t b = 0; b < NB; ++b) {
+    if (b < nbytes) {
+      const uint32_t i = in[b] & 0xffffu, g = (in[b] >> 16) & 255u, y = in[b] >> 24;
+      const uint32_t r = 8 * b + j;
+      if ((cmask >> b) & 1u) {
+        // two bits of one order share a slot: one bit at a time
+        for (uint32_t q = 0; q < 8; ++q) {
+          if (q == j) {
+            const uint32_t v = tv[i];
+            const uint32_t st = v > 255u ? g : v;
+            tv[i] = (uint16_t)T->nex[2 * st + y];
+            rgk[r] = (uint8_t)st;
+          }
+          lane_step();
+        }
+      } else {
         const uint32_t v = tv[i];
         const uint32_t st = v > 255u ? g : v;
         tv[i] = (uint16_t)T->nex[2 * st + y];
         rgk[r] = (uint8_t)st;
+        lane_step();
       }
-      lane_step();
     }
-    i = i_n;
-    g = g_n;
-    y = y_n;
   }
   wave_sync();
 }
@@ -909,7 +928,7 @@ CUCM_DEV void team_enc_issuer(const Task &tk, const TeamMem &X, const LDS DevTab
 #pragma unroll
         for (int k = 0; k < 8; ++k) lds_add(&M.w[cx[j] * 8 + k], dl[j][k]);
     clk.mark(5);
-    if (w == 1) team_resolve_unit(X, T, nvalid / 8, Umax, lane);
+    if (w == 1) team_resolve_unit<8u * NJ>(X, T, nvalid / 8, Umax, lane);
     clk.mark(6);
     team_sync_lds();  // P3
     clk.mark(9);
