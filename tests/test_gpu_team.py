@@ -23,6 +23,10 @@ def _data(kind, n, seed):
         return datagen.structured(n, seed).tobytes()
     if kind == "random":
         return np.random.default_rng(seed).integers(0, 256, n, dtype=np.uint8).tobytes()
+    if kind == "runs":  # runs of 1..39 equal bytes
+        rng = np.random.default_rng(seed)
+        return b"".join(bytes([int(rng.integers(0, 256))]) * int(rng.integers(1, 40))
+                        for _ in range(n // 4 + 1))[:n]
     return bytes(n)
 
 
@@ -44,6 +48,10 @@ CASES = [
     ("markov2", 1, 1, 32, 100, 0, 8, 2),
     ("markov2", 60000, 8192, 128, 3, 7000, 16, 4),  # prelearned initial model
     ("mixed", 3000, 3000, 128, 1, 0, 25, 4),        # reference table size
+    # 2^8 / 2^9 slots: bits of a byte, and of consecutive bytes, share slots
+    ("random", 20000, 4096, 128, 100, 0, 8, 4),
+    ("runs", 30000, 8192, 256, 100, 0, 9, 2),
+    ("zeros", 12000, 4096, 64, 100, 0, 8, 4),
 ]
 
 
