@@ -20,16 +20,18 @@ sys.path.insert(0, ROOT)
 from cucm_amd import datagen  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
-# name, generator, n, seed, S, U, batch, P, tb
+# name, generator, n, seed, S, U, batch, P, tb, frozen model
 FIXTURES = [
-    ("markov2_64k", "markov2", 65536, 1, 16384, 128, 100, 0, 18),
-    ("mixed_32k", "mixed", 32768, 2, 4096, 64, 3, 0, 16),
-    ("structured_24k", "structured", 24576, 4, 8192, 128, 2, 0, 20),
-    ("tail_10b", "markov2", 10, 5, 4096, 128, 100, 0, 18),
-    ("prelearn_8k", "markov2", 8192, 6, 2048, 32, 2, 2048, 18),
-    ("tb25_3000", "markov2", 3000, 7, 3000, 128, 1, 0, 25),
-    ("u256_20k", "mixed", 20000, 8, 5000, 256, 4, 0, 14),
-    ("empty", "markov2", 0, 9, 4096, 128, 100, 0, 18),
+    ("markov2_64k", "markov2", 65536, 1, 16384, 128, 100, 0, 18, False),
+    ("mixed_32k", "mixed", 32768, 2, 4096, 64, 3, 0, 16, False),
+    ("structured_24k", "structured", 24576, 4, 8192, 128, 2, 0, 20, False),
+    ("tail_10b", "markov2", 10, 5, 4096, 128, 100, 0, 18, False),
+    ("prelearn_8k", "markov2", 8192, 6, 2048, 32, 2, 2048, 18, False),
+    ("tb25_3000", "markov2", 3000, 7, 3000, 128, 1, 0, 25, False),
+    ("u256_20k", "mixed", 20000, 8, 5000, 256, 4, 0, 14, False),
+    ("empty", "markov2", 0, 9, 4096, 128, 100, 0, 18, False),
+    ("frozen_prelearn_24k", "markov2", 24576, 10, 4096, 128, 3, 8192, 16, True),
+    ("frozen_zero_5k", "mixed", 5000, 11, 2048, 64, 100, 0, 12, True),
 ]
 
 
@@ -41,9 +43,9 @@ def make_input(gen, n, seed):
 
 def main():
     manifest = []
-    for name, gen, n, seed, S, U, batch, P, tb in FIXTURES:
+    for name, gen, n, seed, S, U, batch, P, tb, frozen in FIXTURES:
         data = make_input(gen, n, seed)
-        blob = O.compress(data, S, U, batch, P, tb)
+        blob = O.compress(data, S, U, batch, P, tb, frozen=frozen)
         assert O.decompress(blob) == data
         with open(os.path.join(HERE, name + ".in"), "wb") as f:
             f.write(data)
@@ -51,7 +53,8 @@ def main():
             f.write(blob)
         manifest.append({"name": name, "generator": gen, "n": n, "seed": seed, "chunk_size": S,
                          "unit_bits": U, "batch_chunks": batch, "prelearn_size": P,
-                         "table_bits": tb, "input_sha256": hashlib.sha256(data).hexdigest(),
+                         "table_bits": tb, "frozen": frozen,
+                         "input_sha256": hashlib.sha256(data).hexdigest(),
                          "container_sha256": hashlib.sha256(blob).hexdigest(),
                          "container_bytes": len(blob)})
         print("%-16s %6d -> %6d bytes" % (name, len(data), len(blob)))

@@ -19,7 +19,7 @@ def test_oracle_reproduces_fixture(fx):
     assert hashlib.sha256(data).hexdigest() == fx["input_sha256"]
     assert hashlib.sha256(blob).hexdigest() == fx["container_sha256"]
     got = O.compress(data, fx["chunk_size"], fx["unit_bits"], fx["batch_chunks"],
-                     fx["prelearn_size"], fx["table_bits"])
+                     fx["prelearn_size"], fx["table_bits"], frozen=fx["frozen"])
     assert got == blob
     assert O.decompress(blob) == data
 

@@ -23,7 +23,7 @@ def test_golden_fixture(ctx, fx):
     data = open(os.path.join(GOLDEN, fx["name"] + ".in"), "rb").read()
     blob = open(os.path.join(GOLDEN, fx["name"] + ".cucm"), "rb").read()
     p = make_params(fx["chunk_size"], fx["unit_bits"], fx["batch_chunks"], fx["prelearn_size"],
-                    fx["table_bits"])
+                    fx["table_bits"], frozen=fx["frozen"])
     assert ctx.compress(data, p) == blob
     assert ctx.decompress(blob) == data
 
