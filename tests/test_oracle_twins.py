@@ -172,3 +172,19 @@ def test_header_layout_reference_fields():
 def test_empty_input_is_header_only():
     blob = O.compress(b"", 4096, 128, 100, 0, 18)
     assert len(blob) == 21 and O.decompress(blob) == b""
+
+
+@pytest.mark.parametrize("n", [1, 11, 12, 999, 1000, 1001, 1999, 2000, 2001])
+def test_edge_sizes_twins_agree_and_round_trip(n):
+    """Around one unit (U = 96: 12 bytes) and one and two chunks (S = 1000;
+    U does not divide 8S): the clean and literal encoders agree, and both
+    decoders (clean, and the restatement of the reference's lock-step
+    decoder) give the input back."""
+    S, U, tb = 1000, 96, 12
+    data = make("text", n, n)
+    m = O.Model(tb)
+    clean = O.encode_batch(data, S, U, m)
+    assert clean == O.encode_batch(data, S, U, m, literal=True)
+    assert O.decode_batch(clean, n, S, U, m) == data
+    back, _ = O.decode_batch(clean, n, S, U, m, literal_m=3)
+    assert back == data
