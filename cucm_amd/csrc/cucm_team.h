@@ -403,39 +403,87 @@ CUCM_DEV void team_resolve_byte(const TeamMem &X, const LDS DevTables *T, uint32
     CUCM_WALK_STEP(7, wyb_ & 1u);                                             \
   } while (0)
 
+// The touched slots of up to three orders ks[0..no) (unit bits r < nbits, a
+// multiple of 8) written back to HBM from the distinct-slot tables, then
+// cleared there: every slot index of the wave goes out together, then every
+// table entry, then the stores, so the wave waits for LDS twice in all.
+// Lanes past nbits repeat the last bit's loads (in bounds) and store nothing.
+CUCM_DEV void writeback_orders(const WaveMem &M, GLB uint8_t *slot, uint32_t tb, uint32_t nbits,
+                               uint32_t Umax, const uint32_t ks[3], uint32_t no,
+                               uint32_t lane) {
+  constexpr int NR = (int)(kMaxUnitBits / 64);
+  const uint32_t TS = table_size(Umax);
+  uint32_t idx[3][NR], key[3][NR], val[3][NR];
+#pragma unroll
+  for (int o = 0; o < 3; ++o)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const uint32_t k = o < (int)no ? ks[o] : ks[0];
+      idx[o][q] = (k - 1) * TS + M.ridx[k * Umax + min(lane + 64u * q, nbits - 1)];
+    }
+#pragma unroll
+  for (int o = 0; o < 3; ++o)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      key[o][q] = M.tkey[idx[o][q]];
+      val[o][q] = M.tval[idx[o][q]];
+    }
+#pragma unroll
+  for (int o = 0; o < 3; ++o)
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      if (o < (int)no && lane + 64u * q < nbits)
+        slot[((uint64_t)(ks[o] - 1) << tb) + key[o][q]] = (uint8_t)val[o][q];
+  wave_sync();
+#pragma unroll
+  for (int o = 0; o < 3; ++o)
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      if (o < (int)no && lane + 64u * q < nbits) {
+        M.tkey[idx[o][q]] = kEmpty;
+        M.tval[idx[o][q]] = kNoState;
+      }
+}
+
 // Unit-end part run by waves 0, 1 and 3 (part 0, 1, 2): the touched slots of
 // a third of the orders written back to HBM and the tables cleared, a third
 // of the mixer deltas folded into the weights.
 CUCM_DEV void team_fold(const TeamMem &X, GLB uint8_t *slot, uint32_t tb, uint32_t nbits,
                         uint32_t Umax, uint32_t part, uint32_t lane) {
   const WaveMem &M = X.M;
-  const uint32_t TS = table_size(Umax);
-#pragma unroll 1
-  for (uint32_t k = 1 + part; k < 8; k += 3) {
-    GLB uint8_t *tab = slot + ((uint64_t)(k - 1) << tb);
-    for (uint32_t r = lane; r < nbits; r += 64) {
-      const uint32_t i = M.ridx[k * Umax + r];
-      tab[M.tkey[(k - 1) * TS + i]] = (uint8_t)M.tval[(k - 1) * TS + i];
-    }
-    wave_sync();
-    for (uint32_t r = lane; r < nbits; r += 64) {
-      const uint32_t i = M.ridx[k * Umax + r];
-      M.tkey[(k - 1) * TS + i] = kEmpty;
-      M.tval[(k - 1) * TS + i] = kNoState;
-    }
+  const uint32_t ks[3] = {1 + part, 4 + part, 7 + part};
+  writeback_orders(M, slot, tb, nbits, Umax, ks, part == 0 ? 3u : 2u, lane);
+  // weights e = part * 64 + lane + 192 q < 640
+  int32_t w[4], d[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t e = min(part * 64 + lane + 192u * q, (uint32_t)(kMixCtx * kOrders - 1));
+    w[q] = M.w[e];
+    d[q] = X.wd[e];
   }
-  for (uint32_t e = part * 64 + lane; e < kMixCtx * kOrders; e += 192) {
-    M.w[e] += X.wd[e];
-    X.wd[e] = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t e = part * 64 + lane + 192u * q;
+    if (e < kMixCtx * kOrders) {
+      M.w[e] = w[q] + d[q];
+      X.wd[e] = 0;
+    }
   }
 }
 
 // The walked statep becomes the next unit's frozen model (read through sm);
-// wave wid does its quarter.
+// wave wid does its quarter, all of its loads in flight together.
 CUCM_DEV void team_refresh_sm(const TeamMem &X, const LDS DevTables *T, uint32_t wid,
                               uint32_t lane) {
-  for (uint32_t i = wid * 64 + lane; i < 8 * 256; i += kTeamWaves * 64)
-    X.sm[i] = T->stretch[X.stpn[i] >> 20];
+  constexpr int NQ = (int)(8 * 256 / (kTeamWaves * 64));
+  uint32_t v[NQ];
+  int32_t st[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = X.stpn[wid * 64 + lane + kTeamWaves * 64 * q];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) st[q] = T->stretch[v[q] >> 20];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) X.sm[wid * 64 + lane + kTeamWaves * 64 * q] = (int16_t)st[q];
 }
 
 // Every role runs the same barrier sequence per unit: one LDS barrier after
