@@ -531,6 +531,9 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
           uint32_t pc[NM];
           paths_finish_team<NM, P>(X, T, (uint32_t)(hist & 255u) >> 5, buf, lane, g, pc);
           clk.mark(9);
+          // wave 3's candidate-ready word: loaded before the decode, so its
+          // latency overlaps it, and acquired only where it is used
+          const uint32_t cflag = (j == 0 && RB > 1) ? flag_peek(&X.ctl[1]) : 0u;
           uint32_t pre = 0;
 #pragma unroll
           for (int i = 0; i < D; ++i) {
@@ -562,9 +565,10 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
           }
           if (r + 1 < R) {
             // wave 3's slot sets for this outcome, if they are complete
-            if (j == 0 && RB > 1 && lane_first(flag_read(&X.ctl[1])) == rid + r + 1)
+            if (j == 0 && RB > 1 && lane_first(cflag) == rid + r + 1) {
+              flag_acquire();  // the candidates written before the word are visible
               paths_issue_cand<NM, P>(M, X.cand, pre, slot, tb, nbuf, lane, g);
-            else if (j + 1 == RB)
+            } else if (j + 1 == RB)
               paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, nbuf, lane, g);
             else
               paths_issue<NM, P>(M, slot, tb, mask, (j + 1) * D, partial, hist, nbuf, lane, g);

@@ -96,6 +96,13 @@ CUCM_DEV void flag_publish(LDS uint32_t *p, uint32_t v) {
 CUCM_DEV uint32_t flag_read(const LDS uint32_t *p) {
   return __hip_atomic_load((uint32_t *)p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// flag_read split in two: the load (its latency can overlap other work) and,
+// once the value is used, the acquire fence that orders the later LDS reads
+// after it.
+CUCM_DEV uint32_t flag_peek(const LDS uint32_t *p) {
+  return __hip_atomic_load((uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+CUCM_DEV void flag_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
 // Low 32 bits of a * b for operands that fit 24-bit signed integers (a
 // full-rate multiply; the 32-bit product is quarter rate).
 CUCM_DEV uint32_t mul_i24(int32_t a, int32_t b) { return (uint32_t)__mul24(a, b); }

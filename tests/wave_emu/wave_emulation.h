@@ -267,6 +267,8 @@ inline void sched_fence() {}
 inline void wait_lgkm() {}
 inline void flag_publish(uint32_t *p, uint32_t v) { *p = v; }
 inline uint32_t flag_read(const uint32_t *p) { return *p; }
+inline uint32_t flag_peek(const uint32_t *p) { return *p; }
+inline void flag_acquire() {}
 inline uint32_t mul_i24(int32_t a, int32_t b) {
   if (a < -(1 << 23) || a >= (1 << 23) || b < -(1 << 23) || b >= (1 << 23)) {
     fprintf(stderr, "wave emulation: mul_i24 operand out of range (%d, %d)\n", a, b);
