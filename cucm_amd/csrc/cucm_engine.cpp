@@ -680,7 +680,7 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
     const uint32_t U = uf & ~31u;
     const uint32_t tb = (uf & 31u) ? (uf & 31u) : 25u;
     const uint64_t segb = flag ? Pc : P;
-    if (P > N || h[20] > (1u | kHeaderFrozen) || tb < 8 || kHeader + segb > J.len) {
+    if (P > N || h[20] > (1u | kHeaderFrozen) || tb < 8 || tb > 25 || kHeader + segb > J.len) {
       J.status = CUCM_E_CORRUPT;
       continue;
     }

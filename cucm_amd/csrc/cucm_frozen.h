@@ -125,7 +125,7 @@ CUCM_DEV void frozen_block_issue(const FrozenMem &F, const GLB uint8_t *states, 
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
         H = (H ^ h[k]) * kFnvPrime;
-        g[j][k] = states[((uint64_t)(k - 1) << tb) + (H & mask)];
+        g[j][k] = states[((uint32_t)(k - 1) << tb) | (H & mask)];
       }
     }
   } else {
@@ -221,12 +221,14 @@ CUCM_DEV void frozen_paths_issue(const FrozenMem &F, const GLB uint8_t *states, 
     const uint32_t h0 = ((1u << (j0 + plen)) | (partial << plen) | pbits) & 255u;
     g[m][0] = F.st0[h0];
     if (states) {
+      const uint32_t lmask = ok ? mask : 0u;
       uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
         H = (H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u)) * kFnvPrime;
-        // lanes without a path read the table's first byte
-        g[m][k] = states[ok ? ((uint64_t)(k - 1) << tb) + (H & mask) : 0ull];
+        // lanes without a path read the table's first byte; the offset
+        // (k - 1) * 2^tb + slot < 7 * 2^25 fits 32 bits
+        g[m][k] = states[((uint32_t)(k - 1) << tb) | (H & lmask)];
       }
     } else {
 #pragma unroll
@@ -273,7 +275,7 @@ CUCM_DEV void frozen_decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
   GLB uint8_t *dst = (GLB uint8_t *)(uintptr_t)tk.dst;
   uint64_t winbase = 0;
   uint32_t win = load_window(pay, plen, 0, lane);
-  uint32_t x = __builtin_bswap32(lane_read(win, 0));
+  uint32_t x = lane_read(win, 0);
   uint32_t x1 = 0, x2 = 0xffffffffu;
   uint64_t ip = 4;
   uint64_t hist = 0;     // the 8 bytes before the current one, latest lowest
@@ -295,11 +297,8 @@ CUCM_DEV void frozen_decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
       const uint32_t q = (1u << i) - 1u + pre;
       const uint32_t pr = path_value<NM>(pc, q);
       const uint32_t xmid = coder_mid(x1, x2, pr);
-      const uint32_t y = x <= xmid ? 1u : 0u;
-      if (y)
-        x2 = xmid;
-      else
-        x1 = xmid + 1u;
+      const uint32_t y = coder_split(x, xmid, x1, x2);
+      pre = (pre << 1) | y;
       if (((x1 ^ x2) & 0xff000000u) == 0) {
         // shift out every leading byte x1 and x2 agree on (at most 4)
         const uint32_t nb = min((uint32_t)clz32(x1 ^ x2) >> 3, 4u);
@@ -310,7 +309,6 @@ CUCM_DEV void frozen_decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
         x = (uint32_t)((((uint64_t)x << 32) | nxt) >> (32 - sh));
         ip += nb;
       }
-      pre = (pre << 1) | y;
     }
     partial = (partial << D) | pre;
     if (((rb & 7u) + D) == 8) {

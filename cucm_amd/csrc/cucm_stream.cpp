@@ -124,7 +124,7 @@ int cucm_stream_decompress_begin(cucm_ctx *ctx, const uint8_t *head, uint64_t he
   rc = cucm_peek_header(head, head_len, &N, &hp);
   if (rc) return rc;
   const uint64_t P = hp.prelearn_size;
-  if (P > N) return CUCM_E_CORRUPT;
+  if (P > N || hp.table_bits < 8 || hp.table_bits > 25) return CUCM_E_CORRUPT;
   if (N - P > 0 && (hp.chunk_size == 0 || hp.unit_bits == 0 || hp.batch_chunks == 0))
     return CUCM_E_CORRUPT;
   if (hp.unit_bits > kMaxUnitBits) return CUCM_E_UNSUPPORTED;

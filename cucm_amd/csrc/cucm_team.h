@@ -203,9 +203,10 @@ CUCM_DEV void paths_issue_cand(const WaveMem &M, const LDS uint32_t *cand, uint3
     const u32x4 c0 = c[0], c1 = c[1];
     const uint32_t sl[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
     g[m][0] = M.st0[sl[0]];
+    // lanes without a path load path 0's slots (in bounds, same lines); the
+    // offset (k - 1) * 2^tb + slot < 7 * 2^25 fits 32 bits
 #pragma unroll
-    for (int k = 1; k < 8; ++k)
-      g[m][k] = slot[ok ? ((uint64_t)(k - 1) << tb) + sl[k] : 0ull];
+    for (int k = 1; k < 8; ++k) g[m][k] = slot[((uint32_t)(k - 1) << tb) | sl[k]];
     if (ok) {
       LDS u32x4 *d = (LDS u32x4 *)(ps + q * 8u);
       d[0] = c0;
@@ -433,7 +434,7 @@ CUCM_DEV void writeback_orders(const WaveMem &M, GLB uint8_t *slot, uint32_t tb,
 #pragma unroll
     for (int q = 0; q < NR; ++q)
       if (o < (int)no && lane + 64u * q < nbits)
-        slot[((uint64_t)(ks[o] - 1) << tb) + key[o][q]] = (uint8_t)val[o][q];
+        slot[((ks[o] - 1) << tb) | key[o][q]] = (uint8_t)val[o][q];
   wave_sync();
 #pragma unroll
   for (int o = 0; o < 3; ++o)
@@ -509,7 +510,7 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
   const uint32_t L = tk.L;
   uint32_t winbase = 0;
   uint32_t win = load_window(pay, plen, 0, lane);
-  uint32_t x = __builtin_bswap32(lane_read(win, 0)), x1 = 0, x2 = 0xffffffffu;
+  uint32_t x = lane_read(win, 0), x1 = 0, x2 = 0xffffffffu;
   uint32_t ip = 4;
   uint64_t hist = 0;  // the 8 bytes before the current one, latest lowest
   uint32_t g[NM][8];
@@ -540,11 +541,8 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
             const uint32_t q = (1u << i) - 1u + pre;
             const uint32_t pr = lane_read(pc[0], q) & 0xffffu;
             const uint32_t xmid = coder_mid(x1, x2, pr);
-            const uint32_t y = x <= xmid ? 1u : 0u;
-            if (y)
-              x2 = xmid;
-            else
-              x1 = xmid + 1u;
+            const uint32_t y = coder_split(x, xmid, x1, x2);
+            pre = (pre << 1) | y;
             if (((x1 ^ x2) & 0xff000000u) == 0) {
               // shift out every leading byte x1 and x2 agree on (at most 4)
               const uint32_t nb = min((uint32_t)clz32(x1 ^ x2) >> 3, 4u);
@@ -555,7 +553,6 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
               x = (uint32_t)((((uint64_t)x << 32) | nxt) >> (32 - sh));
               ip += nb;
             }
-            pre = (pre << 1) | y;
           }
           clk.mark(10);
           partial = (partial << D) | pre;

@@ -52,6 +52,23 @@ CUCM_DEV void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0f70); }    // vmcnt(0)
 // Keeps the instruction scheduler from moving code across this point.
 CUCM_DEV void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 CUCM_DEV void wait_lgkm() { __builtin_amdgcn_s_waitcnt(0xc07f); }  // lgkmcnt(0)
+// One decoded bit on the scalar unit (all operands wave-uniform): y = x <=
+// xmid, then x2 = xmid if y, else x1 = xmid + 1; returns y as an integer.
+// Written as a compare, the flag outlives the refill branch and is moved
+// back to the scalar unit through a vector register on every bit.
+CUCM_DEV uint32_t coder_split(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t &x2) {
+  const uint32_t up = xmid + 1u;
+  uint32_t y;
+  asm volatile(
+      "s_cmp_le_u32 %3, %4\n\t"
+      "s_cselect_b32 %0, 1, 0\n\t"
+      "s_cselect_b32 %2, %4, %2\n\t"
+      "s_cselect_b32 %1, %1, %5"
+      : "=&s"(y), "+s"(x1), "+s"(x2)
+      : "s"(x), "s"(xmid), "s"(up)
+      : "scc");
+  return y;
+}
 // Orders the wave's LDS traffic across lanes (and stops the compiler from
 // moving LDS accesses across it); lanes of a wave run in lock step.
 CUCM_DEV void wave_sync() {
@@ -597,7 +614,7 @@ CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
     }
     g[j][0] = M.st0[h0];
 #pragma unroll
-    for (int k = 1; k < 8; ++k) g[j][k] = slot[((uint64_t)(k - 1) << tb) + sl[j][k]];
+    for (int k = 1; k < 8; ++k) g[j][k] = slot[((uint32_t)(k - 1) << tb) | sl[j][k]];
   }
   // distinct-slot tables: the first probes of every key go out together;
   // lanes past the valid bits probe a dummy word; keys that meet another
@@ -802,7 +819,8 @@ CUCM_DEV void encode_task(const CodecArgs &a, const Task &tk, uint32_t t,
 // ---------------------------------------------------------------------------
 
 // Window of the payload in registers: lane l holds bytes 4l..4l+3 after
-// winbase (0xff past the payload, as the reference decoder reads).
+// winbase as a big-endian word (0xff past the payload, as the reference
+// decoder reads), so the coder's refill is a scalar funnel shift.
 CUCM_DEV uint32_t load_window(const GLB uint8_t *pay, uint64_t plen, uint64_t wb,
                                                 uint32_t lane) {
   if (plen == 0) return 0xffffffffu;
@@ -818,7 +836,7 @@ CUCM_DEV uint32_t load_window(const GLB uint8_t *pay, uint64_t plen, uint64_t wb
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint64_t at = wb + 4ull * lane + i;
-    w |= (at < plen ? b[i] : 255u) << (8 * i);
+    w |= (at < plen ? b[i] : 255u) << (24 - 8 * i);
   }
   return w;
 }
@@ -827,8 +845,7 @@ CUCM_DEV uint32_t load_window(const GLB uint8_t *pay, uint64_t plen, uint64_t wb
 CUCM_DEV uint32_t window_be32(uint32_t win, uint32_t off) {
   const uint32_t w0 = lane_read(win, off >> 2);
   const uint32_t w1 = lane_read(win, ((off >> 2) + 1u) & 63u);
-  const uint64_t v = (((uint64_t)w1 << 32) | w0) >> (8 * (off & 3u));
-  return __builtin_bswap32((uint32_t)v);
+  return (uint32_t)(((((uint64_t)w0 << 32) | w1) << (8 * (off & 3u))) >> 32);
 }
 
 // Speculative paths of one decoder round of D bits: lane q + 64m evaluates
@@ -849,22 +866,23 @@ CUCM_DEV void paths_issue(const WaveMem &M, const GLB uint8_t *slot, uint32_t tb
     const bool ok = q < P;
     const uint32_t plen = 31u - clz32(q + 1u);
     const uint32_t pbits = q + 1u - (1u << plen);
-    // (lanes without a path compute in-bounds garbage)
+    // (lanes without a path compute in-bounds garbage, and slot 0)
     const uint32_t h0 = ((1u << (j0 + plen)) | (partial << plen) | pbits) & 255u;
+    const uint32_t lmask = ok ? mask : 0u;
     uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
     uint32_t sl[8];
     sl[0] = h0;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
       H = (H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u)) * kFnvPrime;
-      sl[k] = H & mask;
+      sl[k] = H & lmask;
     }
     g[m][0] = M.st0[h0];
     // unconditional loads (a predicated load would be waited for at the
-    // branch join); lanes without a path all read the slot's first byte
+    // branch join); lanes without a path all read each table's first byte.
+    // The offset (k - 1) * 2^tb + slot < 7 * 2^25 fits 32 bits.
 #pragma unroll
-    for (int k = 1; k < 8; ++k)
-      g[m][k] = slot[ok ? ((uint64_t)(k - 1) << tb) + sl[k] : 0ull];
+    for (int k = 1; k < 8; ++k) g[m][k] = slot[((uint32_t)(k - 1) << tb) | sl[k]];
     if (ok) {
       LDS u32x4 *d = (LDS u32x4 *)(ps + q * 8u);
       d[0] = u32x4{sl[0], sl[1], sl[2], sl[3]};
@@ -941,7 +959,7 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
   clk.mark(0);
   uint64_t winbase = 0;
   uint32_t win = load_window(pay, plen, 0, lane);
-  uint32_t x = __builtin_bswap32(lane_read(win, 0));
+  uint32_t x = lane_read(win, 0);
   uint32_t x1 = 0, x2 = 0xffffffffu;
   uint64_t ip = 4;
   uint64_t hist = 0;     // the 8 bytes before the current one, latest lowest
@@ -975,11 +993,8 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
         const uint32_t v = path_value<NM>(pc, q);
         const uint32_t pr = v & 0xffffu, cx = v >> 16;
         const uint32_t xmid = coder_mid(x1, x2, pr);
-        const uint32_t y = x <= xmid ? 1u : 0u;
-        if (y)
-          x2 = xmid;
-        else
-          x1 = xmid + 1u;
+        const uint32_t y = coder_split(x, xmid, x1, x2);
+        pre = (pre << 1) | y;
         if (((x1 ^ x2) & 0xff000000u) == 0) {
           // shift out every leading byte x1 and x2 agree on (at most 4)
           const uint32_t nb = min((uint32_t)clz32(x1 ^ x2) >> 3, 4u);
@@ -990,7 +1005,6 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
           x = (uint32_t)((((uint64_t)x << 32) | nxt) >> (32 - sh));
           ip += nb;
         }
-        pre = (pre << 1) | y;
         chosen |= (uint64_t)q << (8 * i);
         prmp[i >> 2] |= (uint64_t)((pr << 1) | y) << (16 * (i & 3));
         cxp |= (uint64_t)cx << (8 * i);

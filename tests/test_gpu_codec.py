@@ -117,7 +117,10 @@ def test_no_tail_pad_flag(ctx, oracle):
 def test_corrupt_and_truncated_containers_are_reported(ctx):
     data = datagen.markov2(30000, 24).tobytes()
     blob = ctx.compress(data, make_params(4096, 128, 100, 0, 18))
-    bad = [blob[:10], blob[:200], blob[:21] + b"\xff" * 40 + blob[61:]]
+    uf = (blob[6] << 8 | blob[7]) & ~31
+    # table bits outside 8..25 (no encoder writes them)
+    tbs = [blob[:6] + (uf | t).to_bytes(2, "big") + blob[8:] for t in (3, 26)]
+    bad = [blob[:10], blob[:200], blob[:21] + b"\xff" * 40 + blob[61:]] + tbs
     n = len(bad)
     import ctypes
     from cucm_amd._native import lib
@@ -130,7 +133,7 @@ def test_corrupt_and_truncated_containers_are_reported(ctx):
     st = (ctypes.c_int32 * n)()
     rc = lib().cucm_decompress_many(ctx.handle, n, ins, lens, outp, caps, olen, st, 4)
     assert rc == 0
-    assert list(st) == [-2, -2, -2]
+    assert list(st) == [-2] * n
     small = (ctypes.c_uint64 * 1)(100)
     ins1 = (ctypes.c_void_p * 1)(ctypes.cast(ctypes.c_char_p(blob), ctypes.c_void_p).value)
     lens1 = (ctypes.c_uint64 * 1)(len(blob))

@@ -64,6 +64,13 @@ def test_stream_rejects_misaligned_segments_and_partial_records(ctx):
     assert lib().cucm_stream_decompress_next(s, rec, len(rec), out.ctypes.data, out.size,
                                              ctypes.byref(olen)) == -2
     lib().cucm_stream_end(s)
+    # table bits outside 8..25 in the header
+    uf = (blob[6] << 8 | blob[7]) & ~31
+    for t in (3, 26):
+        bad = blob[:6] + (uf | t).to_bytes(2, "big") + blob[8:21]
+        assert lib().cucm_stream_decompress_begin(ctx.handle, bad, 21, 4, ctypes.byref(n0),
+                                                  ctypes.byref(q), None, 0,
+                                                  ctypes.byref(s)) == -2
 
 
 def test_stream_refuses_sizes_beyond_the_container_fields(ctx):

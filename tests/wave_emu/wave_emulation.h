@@ -265,6 +265,14 @@ inline uint64_t shader_clock() { return 0; }
 inline void wait_vm() {}
 inline void sched_fence() {}
 inline void wait_lgkm() {}
+inline uint32_t coder_split(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t &x2) {
+  const uint32_t y = x <= xmid ? 1u : 0u;
+  if (y)
+    x2 = xmid;
+  else
+    x1 = xmid + 1u;
+  return y;
+}
 inline void flag_publish(uint32_t *p, uint32_t v) { *p = v; }
 inline uint32_t flag_read(const uint32_t *p) { return *p; }
 inline uint32_t flag_peek(const uint32_t *p) { return *p; }
