@@ -299,7 +299,7 @@ CUCM_DEV void frozen_decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
       const uint32_t xmid = coder_mid(x1, x2, pr);
       const uint32_t y = coder_split(x, xmid, x1, x2);
       pre = (pre << 1) | y;
-      if (((x1 ^ x2) & 0xff000000u) == 0) {
+      if (__builtin_expect(((x1 ^ x2) & 0xff000000u) == 0, 0)) {
         // shift out every leading byte x1 and x2 agree on (at most 4)
         const uint32_t nb = min((uint32_t)clz32(x1 ^ x2) >> 3, 4u);
         const uint32_t sh = 8 * nb;

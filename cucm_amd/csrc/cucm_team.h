@@ -535,15 +535,16 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
           // wave 3's candidate-ready word: loaded before the decode, so its
           // latency overlaps it, and acquired only where it is used
           const uint32_t cflag = (j == 0 && RB > 1) ? flag_peek(&X.ctl[1]) : 0u;
-          uint32_t pre = 0;
+          const uint32_t prv = pc[0] & 0xffffu;
+          // q: the path of the next bit, 2^i - 1 + (the round's bits so far)
+          uint32_t q = 0;
 #pragma unroll
           for (int i = 0; i < D; ++i) {
-            const uint32_t q = (1u << i) - 1u + pre;
-            const uint32_t pr = lane_read(pc[0], q) & 0xffffu;
+            const uint32_t pr = lane_read(prv, q);
             const uint32_t xmid = coder_mid(x1, x2, pr);
-            const uint32_t y = coder_split(x, xmid, x1, x2);
-            pre = (pre << 1) | y;
-            if (((x1 ^ x2) & 0xff000000u) == 0) {
+            coder_split_path(x, xmid, x1, x2, q);
+            // (the refill is the rare case: its code goes out of line)
+            if (__builtin_expect(((x1 ^ x2) & 0xff000000u) == 0, 0)) {
               // shift out every leading byte x1 and x2 agree on (at most 4)
               const uint32_t nb = min((uint32_t)clz32(x1 ^ x2) >> 3, 4u);
               const uint32_t sh = 8 * nb;
@@ -554,6 +555,7 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
               ip += nb;
             }
           }
+          const uint32_t pre = q - P;  // the round's D bits
           clk.mark(10);
           partial = (partial << D) | pre;
           if (j + 1 == RB) {

@@ -69,6 +69,22 @@ CUCM_DEV uint32_t coder_split(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t 
       : "scc");
   return y;
 }
+// coder_split that advances a speculative path index instead of returning
+// the bit: q = 2q + 1 + y (the path of the next bit of the round).
+CUCM_DEV void coder_split_path(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t &x2,
+                               uint32_t &q) {
+  const uint32_t up = xmid + 1u;
+  uint32_t t;
+  asm volatile(
+      "s_cmp_le_u32 %4, %5\n\t"
+      "s_cselect_b32 %0, 2, 1\n\t"
+      "s_cselect_b32 %2, %5, %2\n\t"
+      "s_cselect_b32 %1, %1, %6\n\t"
+      "s_lshl1_add_u32 %3, %3, %0"
+      : "=&s"(t), "+s"(x1), "+s"(x2), "+s"(q)
+      : "s"(x), "s"(xmid), "s"(up)
+      : "scc");
+}
 // Orders the wave's LDS traffic across lanes (and stops the compiler from
 // moving LDS accesses across it); lanes of a wave run in lock step.
 CUCM_DEV void wave_sync() {
@@ -995,7 +1011,7 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
         const uint32_t xmid = coder_mid(x1, x2, pr);
         const uint32_t y = coder_split(x, xmid, x1, x2);
         pre = (pre << 1) | y;
-        if (((x1 ^ x2) & 0xff000000u) == 0) {
+        if (__builtin_expect(((x1 ^ x2) & 0xff000000u) == 0, 0)) {
           // shift out every leading byte x1 and x2 agree on (at most 4)
           const uint32_t nb = min((uint32_t)clz32(x1 ^ x2) >> 3, 4u);
           const uint32_t sh = 8 * nb;

@@ -273,6 +273,9 @@ inline uint32_t coder_split(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t &x
     x1 = xmid + 1u;
   return y;
 }
+inline void coder_split_path(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t &x2, uint32_t &q) {
+  q = 2 * q + 1 + coder_split(x, xmid, x1, x2);
+}
 inline void flag_publish(uint32_t *p, uint32_t v) { *p = v; }
 inline uint32_t flag_read(const uint32_t *p) { return *p; }
 inline uint32_t flag_peek(const uint32_t *p) { return *p; }
