@@ -215,6 +215,48 @@ CUCM_DEV void paths_issue_cand(const WaveMem &M, const LDS uint32_t *cand, uint3
   }
 }
 
+// paths_issue of a byte's first round (j0 = 0) in two parts: the gathers go
+// out before the leader's byte barrier, the LDS part (order-0 state, slot
+// record) follows it, so the barrier's LDS wait is only the byte's own
+// stores.
+template <int NM, uint32_t P>
+CUCM_DEV void paths_issue_gathers(const GLB uint8_t *slot, uint32_t tb, uint32_t mask,
+                                  uint64_t hist, uint32_t lane, uint32_t g[NM][8],
+                                  uint32_t sl[NM][8]) {
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const uint32_t q = lane + 64u * m;
+    const uint32_t plen = 31u - clz32(q + 1u);
+    const uint32_t pbits = q + 1u - (1u << plen);
+    const uint32_t h0 = ((1u << plen) | pbits) & 255u;
+    const uint32_t lmask = q < P ? mask : 0u;
+    uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
+    sl[m][0] = h0;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      H = (H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u)) * kFnvPrime;
+      sl[m][k] = H & lmask;
+    }
+#pragma unroll
+    for (int k = 1; k < 8; ++k) g[m][k] = slot[((uint32_t)(k - 1) << tb) | sl[m][k]];
+  }
+}
+template <int NM, uint32_t P>
+CUCM_DEV void paths_issue_record(const WaveMem &M, uint32_t buf, uint32_t lane,
+                                 const uint32_t sl[NM][8], uint32_t g[NM][8]) {
+  LDS uint32_t *ps = M.pslot + buf * (P * 8u);
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const uint32_t q = lane + 64u * m;
+    g[m][0] = M.st0[sl[m][0]];
+    if (q < P) {
+      LDS u32x4 *d = (LDS u32x4 *)(ps + q * 8u);
+      d[0] = u32x4{sl[m][0], sl[m][1], sl[m][2], sl[m][3]};
+      d[1] = u32x4{sl[m][4], sl[m][5], sl[m][6], sl[m][7]};
+    }
+  }
+}
+
 // paths_finish for the team: the stretch of each order's statep comes from
 // the unit's sm table in one lookup; pc[m] = pr | cxt << 16 of path q also
 // goes to the round's pcb buffer for wave 1.
@@ -519,6 +561,8 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
   for (uint32_t base = 0; base < L; base += U / 8) {
     const uint32_t nblk = min(U / 8, L - base), R = nblk * RB;
     for (uint32_t b = 0; b <= nblk; ++b, ++it) {
+      uint32_t sln[NM][8];  // the next byte's first-round slots (recorded after the barrier)
+      bool next = false;
       if (b < nblk) {
         uint32_t partial = 0;
 #pragma unroll
@@ -567,9 +611,10 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
             if (j == 0 && RB > 1 && lane_first(cflag) == rid + r + 1) {
               flag_acquire();  // the candidates written before the word are visible
               paths_issue_cand<NM, P>(M, X.cand, pre, slot, tb, nbuf, lane, g);
-            } else if (j + 1 == RB)
-              paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, nbuf, lane, g);
-            else
+            } else if (j + 1 == RB) {
+              paths_issue_gathers<NM, P>(slot, tb, mask, hist, lane, g, sln);
+              next = true;
+            } else
               paths_issue<NM, P>(M, slot, tb, mask, (j + 1) * D, partial, hist, nbuf, lane, g);
           }
           clk.mark(1);
@@ -584,6 +629,7 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
       if (b == nblk)
         for (uint32_t i = lane; i < nblk; i += 64) dst[base + i] = M.ub[8 + i];
       team_sync_lds();
+      if (next) paths_issue_record<NM, P>(M, (ubuf + (b + 1) * RB) % NB, lane, sln, g);
       clk.mark(4);
     }
     team_fold(X, slot, tb, 8 * nblk, Umax, 0, lane);
