@@ -52,7 +52,8 @@ __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a) {
 
 // The team decoder: one chunk per workgroup of kTeamWaves waves, each
 // workgroup owning one model slot and pulling chunk tasks from the queue.
-template <int NJ, int D>
+// (kProf: the instantiation with phase clocks, launched when a.prof is set)
+template <int NJ, int D, bool kProf>
 __global__ void __launch_bounds__(256) decode_team_kernel(CodecArgs a) {
   constexpr uint32_t Umax = 64u * NJ;
   extern __shared__ __align__(16) uint8_t lds[];
@@ -72,12 +73,12 @@ __global__ void __launch_bounds__(256) decode_team_kernel(CodecArgs a) {
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
     if (tk.flags & kTaskFrozen) continue;  // uniform: every wave read the same task
-    decode_team<NJ, D>(a, tk, t, X, T, slot, Umax, wid, lane, rid);
+    decode_team<NJ, D, kProf>(a, tk, t, X, T, slot, Umax, wid, lane, rid);
   }
 }
 
 // The team encoder: one chunk per workgroup of kTeamWaves waves.
-template <int NJ>
+template <int NJ, bool kProf>
 __global__ void __launch_bounds__(256) encode_team_kernel(CodecArgs a) {
   constexpr uint32_t Umax = 64u * NJ;
   extern __shared__ __align__(16) uint8_t lds[];
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(256) encode_team_kernel(CodecArgs a) {
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
     if (tk.flags & kTaskFrozen) continue;  // uniform: every wave read the same task
-    encode_team<NJ>(a, tk, t, X, T, slot, Umax, wid, lane);
+    encode_team<NJ, kProf>(a, tk, t, X, T, slot, Umax, wid, lane);
   }
 }
 
@@ -300,27 +301,40 @@ bool team_supported(bool decode, uint32_t spec_bits) {
   return !decode || (spec_bits >= 1 && spec_bits <= kTeamMaxSpec && (8 % spec_bits) == 0);
 }
 
-template <int NJ>
-static hipError_t launch_team_enc(const CodecArgs &a, uint32_t blocks, hipStream_t stream) {
+template <int NJ, bool kProf>
+static hipError_t launch_team_enc_one(const CodecArgs &a, uint32_t blocks, hipStream_t stream) {
   const uint32_t lds = team_codec_lds_bytes(64u * NJ, 0, false);
-  hipError_t e = hipFuncSetAttribute((const void *)encode_team_kernel<NJ>,
+  hipError_t e = hipFuncSetAttribute((const void *)encode_team_kernel<NJ, kProf>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((encode_team_kernel<NJ>), dim3(blocks), dim3(64 * kTeamWaves), lds, stream,
-                     a);
+  hipLaunchKernelGGL((encode_team_kernel<NJ, kProf>), dim3(blocks), dim3(64 * kTeamWaves), lds,
+                     stream, a);
+  return hipGetLastError();
+}
+
+template <int NJ>
+static hipError_t launch_team_enc(const CodecArgs &a, uint32_t blocks, hipStream_t stream) {
+  return a.prof ? launch_team_enc_one<NJ, true>(a, blocks, stream)
+                : launch_team_enc_one<NJ, false>(a, blocks, stream);
+}
+
+template <int NJ, int D, bool kProf>
+static hipError_t launch_team_prof(const CodecArgs &a, uint32_t Umax, uint32_t blocks,
+                                   hipStream_t stream) {
+  const uint32_t lds = team_codec_lds_bytes(Umax, D, true);
+  hipError_t e = hipFuncSetAttribute((const void *)decode_team_kernel<NJ, D, kProf>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((decode_team_kernel<NJ, D, kProf>), dim3(blocks), dim3(64 * kTeamWaves), lds,
+                     stream, a);
   return hipGetLastError();
 }
 
 template <int NJ, int D>
 static hipError_t launch_team_one(const CodecArgs &a, uint32_t Umax, uint32_t blocks,
                                   hipStream_t stream) {
-  const uint32_t lds = team_codec_lds_bytes(Umax, D, true);
-  hipError_t e = hipFuncSetAttribute((const void *)decode_team_kernel<NJ, D>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((decode_team_kernel<NJ, D>), dim3(blocks), dim3(64 * kTeamWaves), lds,
-                     stream, a);
-  return hipGetLastError();
+  return a.prof ? launch_team_prof<NJ, D, true>(a, Umax, blocks, stream)
+                : launch_team_prof<NJ, D, false>(a, Umax, blocks, stream);
 }
 
 template <int NJ>

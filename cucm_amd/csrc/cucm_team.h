@@ -734,12 +734,14 @@ CUCM_DEV void team_hasher(const Task &tk, const TeamMem &X, const LDS DevTables 
 // the leader decodes byte b while wave 1 records and resolves byte b - 1,
 // wave 2 walks byte b - 2 and wave 3 hashes the candidates of byte b's
 // second round.  `rid` numbers rounds across the workgroup's chunks.
-template <int NJ, int D>
+template <int NJ, int D, bool kProf>
 CUCM_DEV void decode_team(const CodecArgs &a, const Task &tk, uint32_t t, const TeamMem &X,
                           const LDS DevTables *T, GLB uint8_t *slot, uint32_t Umax, uint32_t wid,
                           uint32_t lane, uint32_t &rid) {
   static_assert(D >= 1 && D <= (int)kTeamMaxSpec && (8 % D) == 0, "team speculation width");
-  PhaseClock clk(a.prof != nullptr);
+  // (phase clocks only in the profiling instantiation: their registers
+  // would otherwise crowd the leader's)
+  PhaseClock clk(kProf && a.prof != nullptr);
   team_init_model(a, tk, X, T, slot, Umax, wid * 64 + lane);
   team_sync();
   clk.mark(0);
@@ -1114,11 +1116,11 @@ CUCM_DEV void team_enc_coder(const CodecArgs &a, const Task &tk, uint32_t t, con
 }
 
 // One chunk encoded by the team (all kTeamWaves waves call this together).
-template <int NJ>
+template <int NJ, bool kProf>
 CUCM_DEV void encode_team(const CodecArgs &a, const Task &tk, uint32_t t, const TeamMem &X,
                           const LDS DevTables *T, GLB uint8_t *slot, uint32_t Umax, uint32_t wid,
                           uint32_t lane) {
-  PhaseClock clk(a.prof != nullptr);
+  PhaseClock clk(kProf && a.prof != nullptr);
   team_init_enc(a, tk, X, slot, Umax, wid * 64 + lane);
   team_sync();
   clk.mark(0);

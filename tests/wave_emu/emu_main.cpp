@@ -183,7 +183,7 @@ static std::vector<uint8_t> encode_team_all(const std::vector<uint8_t> &in, uint
   a.result = result.data();
   emu::run_team(kTeamWaves, seed, [&](uint32_t wid, uint32_t lane) {
     const TeamMem X = carve_team_enc(base + tab, Umax, wid);
-    for (uint64_t c = 0; c < C; ++c) encode_team<NJ>(a, tasks[c], (uint32_t)c, X, T, slot, Umax, wid, lane);
+    for (uint64_t c = 0; c < C; ++c) encode_team<NJ, true>(a, tasks[c], (uint32_t)c, X, T, slot, Umax, wid, lane);
   });
   std::vector<uint8_t> out;
   for (uint64_t c = 0; c < C; ++c) {
@@ -246,7 +246,7 @@ static void decode_team_all(const std::vector<uint8_t> &in, uint8_t *out, uint64
     const TeamMem X = carve_team(base + tab, U, D, wid);
     uint32_t rid = 0;
     for (uint64_t c = 0; c < C; ++c)
-      decode_team<NJ, D>(a, tasks[c], (uint32_t)c, X, T, slot, U, wid, lane, rid);
+      decode_team<NJ, D, true>(a, tasks[c], (uint32_t)c, X, T, slot, U, wid, lane, rid);
   });
   for (uint64_t c = 0; c < C; ++c) {
     memcpy(out + c * S, dsts[c], tasks[c].L);
