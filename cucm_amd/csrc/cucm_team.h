@@ -257,18 +257,17 @@ CUCM_DEV void paths_issue_record(const WaveMem &M, uint32_t buf, uint32_t lane,
   }
 }
 
-// paths_finish for the team: the stretch of each order's statep comes from
-// the unit's sm table in one lookup; pc[m] = pr | cxt << 16 of path q also
-// goes to the round's pcb buffer for wave 1.
+// paths_finish for the team, in two parts.  The mix: the stretch of each
+// order's statep comes from the unit's sm table in one lookup; pc[m] = pr |
+// cxt << 16 of path q.  The record (the round's unit-start states and pc
+// into the pg / pcb buffers for wave 1) is left for after the decode and the
+// next issue, off the round chain.
 template <int NM, uint32_t P>
-CUCM_DEV void paths_finish_team(const TeamMem &X, const LDS DevTables *T, uint32_t c1,
-                                uint32_t buf, uint32_t lane, const uint32_t g[NM][8],
-                                uint32_t pc[NM]) {
+CUCM_DEV void paths_mix_team(const TeamMem &X, const LDS DevTables *T, uint32_t c1,
+                             const uint32_t g[NM][8], uint32_t pc[NM]) {
   const WaveMem &M = X.M;
-  LDS uint8_t *pgb = M.pg + buf * (P * 8u);
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
-    const uint32_t q = lane + 64u * m;
     int32_t smp[8];
     uint32_t active = 0;
 #pragma unroll
@@ -282,6 +281,15 @@ CUCM_DEV void paths_finish_team(const TeamMem &X, const LDS DevTables *T, uint32
     const int32_t w[8] = {(int32_t)wa.x, (int32_t)wa.y, (int32_t)wa.z, (int32_t)wa.w,
                           (int32_t)wb.x, (int32_t)wb.y, (int32_t)wb.z, (int32_t)wb.w};
     pc[m] = mix_weights(w, smp, T->squash) | (cxt << 16);
+  }
+}
+template <int NM, uint32_t P>
+CUCM_DEV void paths_record_team(const TeamMem &X, uint32_t buf, uint32_t lane,
+                                const uint32_t g[NM][8], const uint32_t pc[NM]) {
+  LDS uint8_t *pgb = X.M.pg + buf * (P * 8u);
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const uint32_t q = lane + 64u * m;
     if (q < P) {
       const uint32_t lo = g[m][0] | (g[m][1] << 8) | (g[m][2] << 16) | (g[m][3] << 24);
       const uint32_t hi = g[m][4] | (g[m][5] << 8) | (g[m][6] << 16) | (g[m][7] << 24);
@@ -555,9 +563,13 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
   uint32_t x = lane_read(win, 0), x1 = 0, x2 = 0xffffffffu;
   uint32_t ip = 4;
   uint64_t hist = 0;  // the 8 bytes before the current one, latest lowest
-  uint32_t g[NM][8];
+  // gathered states, double buffered: round j of a byte mixes g[j & 1] (RB
+  // is even) while the next round's gathers land in the other half, so a
+  // round's record can follow the next issue
+  static_assert(RB % 2 == 0, "rounds per byte");
+  uint32_t g[2][NM][8];
   uint32_t ubuf = 0, it = 0;
-  if (L) paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, ubuf, lane, g);
+  if (L) paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, ubuf, lane, g[0]);
   for (uint32_t base = 0; base < L; base += U / 8) {
     const uint32_t nblk = min(U / 8, L - base), R = nblk * RB;
     for (uint32_t b = 0; b <= nblk; ++b, ++it) {
@@ -574,7 +586,7 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
             win = load_window(pay, plen, winbase, lane);
           }
           uint32_t pc[NM];
-          paths_finish_team<NM, P>(X, T, (uint32_t)(hist & 255u) >> 5, buf, lane, g, pc);
+          paths_mix_team<NM, P>(X, T, (uint32_t)(hist & 255u) >> 5, g[j & 1], pc);
           clk.mark(9);
           // wave 3's candidate-ready word: loaded before the decode, so its
           // latency overlaps it, and acquired only where it is used
@@ -610,13 +622,16 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
             // wave 3's slot sets for this outcome, if they are complete
             if (j == 0 && RB > 1 && lane_first(cflag) == rid + r + 1) {
               flag_acquire();  // the candidates written before the word are visible
-              paths_issue_cand<NM, P>(M, X.cand, pre, slot, tb, nbuf, lane, g);
+              paths_issue_cand<NM, P>(M, X.cand, pre, slot, tb, nbuf, lane, g[(j + 1) & 1]);
             } else if (j + 1 == RB) {
-              paths_issue_gathers<NM, P>(slot, tb, mask, hist, lane, g, sln);
+              paths_issue_gathers<NM, P>(slot, tb, mask, hist, lane, g[0], sln);
               next = true;
-            } else
-              paths_issue<NM, P>(M, slot, tb, mask, (j + 1) * D, partial, hist, nbuf, lane, g);
+            } else {
+              paths_issue<NM, P>(M, slot, tb, mask, (j + 1) * D, partial, hist, nbuf, lane,
+                                 g[(j + 1) & 1]);
+            }
           }
+          paths_record_team<NM, P>(X, buf, lane, g[j & 1], pc);
           clk.mark(1);
         }
       }
@@ -629,7 +644,7 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
       if (b == nblk)
         for (uint32_t i = lane; i < nblk; i += 64) dst[base + i] = M.ub[8 + i];
       team_sync_lds();
-      if (next) paths_issue_record<NM, P>(M, (ubuf + (b + 1) * RB) % NB, lane, sln, g);
+      if (next) paths_issue_record<NM, P>(M, (ubuf + (b + 1) * RB) % NB, lane, sln, g[0]);
       clk.mark(4);
     }
     team_fold(X, slot, tb, 8 * nblk, Umax, 0, lane);
@@ -639,7 +654,7 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
     team_refresh_sm(X, T, 0, lane);
     rid += R;
     ubuf = (ubuf + R) % NB;
-    if (base + U / 8 < L) paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, ubuf, lane, g);
+    if (base + U / 8 < L) paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, ubuf, lane, g[0]);
     team_sync_lds();
     clk.mark(8);
   }
