@@ -39,12 +39,19 @@ constexpr uint32_t kTeamMaxSpec = 4;  // 2^D x (2^D - 1) candidate slot sets fit
 
 // LDS of a team: the chunk's WaveMem (team_round_buffers rounds of path
 // buffers) and:
+// A stretch value of the decoder's per-unit table with its mixer-context
+// flag (stretch != 1, the orders counted as active) in the high half.
+CUCM_HDI uint32_t sm_pack(int32_t st) {
+  return ((uint32_t)st & 0xffffu) | ((uint32_t)(st != 1) << 16);
+}
+CUCM_HDI int32_t sm_stretch(uint32_t v) { return (int32_t)(int16_t)(v & 0xffffu); }
+
 struct TeamMem {
   WaveMem M;
   LDS uint32_t *stpn;  // [8][256] statep being walked (predictions read sm, frozen per unit)
   LDS int32_t *wd;     // [80][8] mixer deltas of the unit
   LDS uint8_t *st0r;   // [256] order-0 states being resolved (M.st0 stays frozen)
-  LDS int16_t *sm;     // [8][256] stretch(statep >> 20) of the frozen statep
+  LDS uint32_t *sm;    // [8][256] stretch(statep >> 20) of the frozen statep, packed by sm_pack
   LDS uint32_t *pcb;   // [NB][P] pr | cxt << 16 of every path of a round, per path buffer
   LDS uint32_t *cand;  // [2^D][P][8] next round's slots per outcome of the current round
   LDS uint8_t *ubw;    // encoder: [Umax / 8] the walked unit's bytes
@@ -75,7 +82,7 @@ CUCM_HDI TeamLayout team_layout(uint32_t Umax, uint32_t D) {
   t.st0r = off;
   off = align16(off + 256);
   t.sm = off;
-  off = align16(off + 8 * 256 * 2);
+  off = align16(off + 8 * 256 * 4);
   t.pcb = off;
   off = align16(off + team_round_buffers(D) * P * 4);
   t.cand = off;
@@ -99,7 +106,7 @@ CUCM_DEV TeamMem carve_team(LDS uint8_t *base, uint32_t Umax, uint32_t D, uint32
   x.stpn = (LDS uint32_t *)(base + l.stpn);
   x.wd = (LDS int32_t *)(base + l.wd);
   x.st0r = base + l.st0r;
-  x.sm = (LDS int16_t *)(base + l.sm);
+  x.sm = (LDS uint32_t *)(base + l.sm);
   x.pcb = (LDS uint32_t *)(base + l.pcb);
   x.cand = (LDS uint32_t *)(base + l.cand);
   x.ubw = nullptr;
@@ -125,7 +132,7 @@ CUCM_DEV void team_init_model(const CodecArgs &a, const Task &tk, const TeamMem 
       const uint32_t v = im->statep[i >> 8][i & 255];
       M.stp[i] = v;
       X.stpn[i] = v;
-      X.sm[i] = T->stretch[v >> 20];
+      X.sm[i] = sm_pack(T->stretch[v >> 20]);
     }
     for (uint32_t i = tid; i < kMixCtx * kOrders; i += NT) M.w[i] = im->w[i];
     for (uint32_t i = tid; i < 256; i += NT) {
@@ -138,7 +145,7 @@ CUCM_DEV void team_init_model(const CodecArgs &a, const Task &tk, const TeamMem 
     for (uint32_t i = tid; i < 8 * 256; i += NT) {
       M.stp[i] = 1u << 31;
       X.stpn[i] = 1u << 31;
-      X.sm[i] = T->stretch[(1u << 31) >> 20];
+      X.sm[i] = sm_pack(T->stretch[(1u << 31) >> 20]);
     }
     for (uint32_t i = tid; i < kMixCtx * kOrders; i += NT) M.w[i] = 0;
     for (uint32_t i = tid; i < 256; i += NT) {
@@ -269,12 +276,15 @@ CUCM_DEV void paths_mix_team(const TeamMem &X, const LDS DevTables *T, uint32_t 
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
     int32_t smp[8];
-    uint32_t active = 0;
+    uint32_t sv[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      smp[k] = X.sm[k * 256 + g[m][k]];
-      active += (smp[k] != 1);
+      sv[k] = X.sm[k * 256 + g[m][k]];
+      smp[k] = sm_stretch(sv[k]);
     }
+    // the active orders (stretch != 1) are the sum of the high halves
+    const uint32_t active = ((sv[0] >> 16) + (sv[1] >> 16)) + ((sv[2] >> 16) + (sv[3] >> 16)) +
+                            ((sv[4] >> 16) + (sv[5] >> 16)) + ((sv[6] >> 16) + (sv[7] >> 16));
     const uint32_t cxt = active + c1 * 10u;
     const LDS u32x4 *wr = (const LDS u32x4 *)(M.w + cxt * 8u);
     const u32x4 wa = wr[0], wb = wr[1];
@@ -371,7 +381,7 @@ CUCM_DEV void team_record_byte(const TeamMem &X, const LDS DevTables *T, uint32_
   M.rg[k * Umax + r] = (uint8_t)gq;
   M.ridx[k * Umax + r] = (uint16_t)(k == 0 ? sl : h);
   const int32_t err = ((int32_t)(y << 12) - (int32_t)(pcv & 0xffffu)) * 7;
-  const int32_t smp = X.sm[k * 256 + gq];
+  const int32_t smp = sm_stretch(X.sm[k * 256 + gq]);
   lds_add(&X.wd[(pcv >> 16) * 8u + k], (int32_t)(((int64_t)err * smp) >> 16));
   wave_sync();
 }
@@ -534,7 +544,7 @@ CUCM_DEV void team_refresh_sm(const TeamMem &X, const LDS DevTables *T, uint32_t
 #pragma unroll
   for (int q = 0; q < NQ; ++q) st[q] = T->stretch[v[q] >> 20];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) X.sm[wid * 64 + lane + kTeamWaves * 64 * q] = (int16_t)st[q];
+  for (int q = 0; q < NQ; ++q) X.sm[wid * 64 + lane + kTeamWaves * 64 * q] = sm_pack(st[q]);
 }
 
 // Every role runs the same barrier sequence per unit: one LDS barrier after
