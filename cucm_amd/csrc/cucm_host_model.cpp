@@ -30,7 +30,9 @@ void build_tables(DevTables *t) {
   }
   st[4095] = 2047;
   for (int i = 0; i < 4096; ++i) {
-    t->squash[i] = (uint16_t)sq[i];
+    // every prediction bumps pr < 2048 by one after the squash
+    // (cucm_compress_v2.cu:787-788): the table holds the bumped value
+    t->squash[i] = (uint16_t)(sq[i] + (sq[i] < 2048 ? 1 : 0));
     t->stretch[i] = (int16_t)st[i];
   }
   for (int n = 0; n < 1024; ++n) t->dt[n] = (uint16_t)(16384 / (2 * n + 3));
@@ -89,9 +91,7 @@ void eval(const HostModel &m, const DevTables &T, const uint8_t *buf, uint64_t p
   for (int k = 0; k < kOrders; ++k) dot += (int64_t)m.w[e.cxt * 8 + k] * e.smp[k];
   dot >>= 16;
   const int idx = dot < -2047 ? -2047 : (dot > 2048 ? 2048 : (int)dot);
-  uint32_t pr = T.squash[idx + 2047];
-  if (pr < 2048) ++pr;
-  e.pr = pr;
+  e.pr = T.squash[idx + 2047];  // the pr bump is in the table
 }
 
 void update(HostModel &m, const DevTables &T, const BitEval &e, uint32_t y) {

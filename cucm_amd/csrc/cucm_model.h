@@ -64,7 +64,7 @@ static const int kSquashKnots[33] = {1,    2,    3,    6,    10,   16,   27,
 // Lookup tables the kernels stage into LDS once per workgroup.
 struct DevTables {
   int16_t stretch[4096];  // ln(p/(1-p)) domain, inverse of squash
-  uint16_t squash[4096];  // 12-bit probability of stretch value d-2047
+  uint16_t squash[4096];  // 12-bit probability of stretch value d-2047, pr bump included
   uint16_t dt[1024];      // 16384 / (2n + 3): StateMap adaptation rate
   uint8_t nex[512];       // nex[2s + y]
 };

@@ -287,12 +287,10 @@ CUCM_DEV int32_t mix_dot(const int32_t w[8], const int32_t smp[8]) {
 }
 
 // squash of the clamped dot product plus the pr bump (cucm_compress_v2.cu:
-// 641-644, 787-788).
+// 641-644, 787-788; the table holds the bumped values).
 CUCM_DEV uint32_t mix_squash(int32_t dot, const LDS uint16_t *squash) {
   const int idx = dot < -2047 ? -2047 : (dot > 2048 ? 2048 : dot);
-  uint32_t pr = squash[idx + 2047];
-  if (pr < 2048) ++pr;
-  return pr;
+  return squash[idx + 2047];
 }
 
 // Mixer::p (cucm_compress_v2.cu:636-644) plus the pr bump (:787-788).
