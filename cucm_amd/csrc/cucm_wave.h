@@ -69,6 +69,18 @@ CUCM_DEV uint32_t coder_split(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t 
       : "scc");
   return y;
 }
+// The encoder's interval update on the scalar unit: the bit is bit 0 of prm
+// (pr << 1 | y); x2 = xmid if it is set, else x1 = xmid + 1.
+CUCM_DEV void coder_encode_split(uint32_t prm, uint32_t xmid, uint32_t &x1, uint32_t &x2) {
+  const uint32_t up = xmid + 1u;
+  asm volatile(
+      "s_bitcmp1_b32 %2, 0\n\t"
+      "s_cselect_b32 %1, %3, %1\n\t"
+      "s_cselect_b32 %0, %0, %4"
+      : "+s"(x1), "+s"(x2)
+      : "s"(prm), "s"(xmid), "s"(up)
+      : "scc");
+}
 // coder_split that advances a speculative path index instead of returning
 // the bit: q = 2q + 1 + y (the path of the next bit of the round).
 CUCM_DEV void coder_split_path(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t &x2,
@@ -731,11 +743,8 @@ CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
   do {                                                                    \
     const uint32_t prm_ = (prm_expr);                                     \
     const uint32_t xmid_ = coder_mid(x1, x2, prm_ >> 1);                  \
-    if (prm_ & 1u)                                                        \
-      x2 = xmid_;                                                         \
-    else                                                                  \
-      x1 = xmid_ + 1u;                                                    \
-    if (((x1 ^ x2) & 0xff000000u) == 0) {                                 \
+    coder_encode_split(prm_, xmid_, x1, x2);                              \
+    if (__builtin_expect(((x1 ^ x2) & 0xff000000u) == 0, 0)) {            \
       /* (rare) shift out the leading bytes x1 and x2 agree on */         \
       do {                                                                \
         acc |= (x2 >> 24) << (8 * (opos & 3u));                           \

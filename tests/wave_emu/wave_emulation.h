@@ -273,6 +273,12 @@ inline uint32_t coder_split(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t &x
     x1 = xmid + 1u;
   return y;
 }
+inline void coder_encode_split(uint32_t prm, uint32_t xmid, uint32_t &x1, uint32_t &x2) {
+  if (prm & 1u)
+    x2 = xmid;
+  else
+    x1 = xmid + 1u;
+}
 inline void coder_split_path(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t &x2, uint32_t &q) {
   q = 2 * q + 1 + coder_split(x, xmid, x1, x2);
 }
