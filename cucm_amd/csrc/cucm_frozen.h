@@ -136,7 +136,7 @@ CUCM_DEV void frozen_block_issue(const FrozenMem &F, const GLB uint8_t *states, 
   }
 }
 
-// prm[j] = pr << 1 | y of bit j of the lane's byte: stretch of each order's
+// prm[j] = pr << 20 | y of bit j of the lane's byte: stretch of each order's
 // prediction, mixer context (c1 >> 5) * 10 + active orders, mix, pr bump.
 CUCM_DEV void frozen_block_mix(const FrozenMem &F, const LDS DevTables *T, uint32_t c,
                                uint32_t c1, const uint32_t g[8][8], uint32_t prm[8]) {
@@ -155,7 +155,7 @@ CUCM_DEV void frozen_block_mix(const FrozenMem &F, const LDS DevTables *T, uint3
     const int32_t w[8] = {(int32_t)wa.x, (int32_t)wa.y, (int32_t)wa.z, (int32_t)wa.w,
                           (int32_t)wb.x, (int32_t)wb.y, (int32_t)wb.z, (int32_t)wb.w};
     const uint32_t pr = mix_weights(w, smp, T->squash);
-    prm[j] = (pr << 1) | ((c >> (7 - j)) & 1u);
+    prm[j] = (pr << 20) | ((c >> (7 - j)) & 1u);
   }
 }
 
@@ -291,12 +291,14 @@ CUCM_DEV void frozen_decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
     }
     uint32_t pc[NM];
     frozen_paths_mix<NM, P>(F, T, (uint32_t)(hist & 255u) >> 5, g, pc);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) pc[m] <<= 20;  // pr << 20 (coder_mid20)
     uint32_t pre = 0;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       const uint32_t q = (1u << i) - 1u + pre;
       const uint32_t pr = path_value<NM>(pc, q);
-      const uint32_t xmid = coder_mid(x1, x2, pr);
+      const uint32_t xmid = coder_mid20(x1, x2, pr);
       const uint32_t y = coder_split(x, xmid, x1, x2);
       pre = (pre << 1) | y;
       if (__builtin_expect(((x1 ^ x2) & 0xff000000u) == 0, 0)) {

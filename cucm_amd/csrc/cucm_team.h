@@ -601,13 +601,13 @@ CUCM_DEV void team_leader(const CodecArgs &a, const Task &tk, uint32_t t, const 
           // wave 3's candidate-ready word: loaded before the decode, so its
           // latency overlaps it, and acquired only where it is used
           const uint32_t cflag = (j == 0 && RB > 1) ? flag_peek(&X.ctl[1]) : 0u;
-          const uint32_t prv = pc[0] & 0xffffu;
+          const uint32_t prv = (pc[0] & 0xffffu) << 20;  // pr << 20 (coder_mid20)
           // q: the path of the next bit, 2^i - 1 + (the round's bits so far)
           uint32_t q = 0;
 #pragma unroll
           for (int i = 0; i < D; ++i) {
             const uint32_t pr = lane_read(prv, q);
-            const uint32_t xmid = coder_mid(x1, x2, pr);
+            const uint32_t xmid = coder_mid20(x1, x2, pr);
             coder_split_path(x, xmid, x1, x2, q);
             // (the refill is the rare case: its code goes out of line)
             if (__builtin_expect(((x1 ^ x2) & 0xff000000u) == 0, 0)) {
@@ -1104,7 +1104,7 @@ CUCM_DEV void team_enc_coder(const CodecArgs &a, const Task &tk, uint32_t t, con
     if (u) {
       uint32_t pv[NJ];
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) pv[j] = M.rprm[lane + 64u * j];
+      for (int j = 0; j < NJ; ++j) pv[j] = prm20(M.rprm[lane + 64u * j]);
       // coded bits are a multiple of 8: blocks of 8 with constant lane offsets
       const uint32_t nc = E.ncoded(u - 1);
 #pragma unroll

@@ -70,7 +70,7 @@ CUCM_DEV uint32_t coder_split(uint32_t x, uint32_t xmid, uint32_t &x1, uint32_t 
   return y;
 }
 // The encoder's interval update on the scalar unit: the bit is bit 0 of prm
-// (pr << 1 | y); x2 = xmid if it is set, else x1 = xmid + 1.
+// (pr << 20 | y); x2 = xmid if it is set, else x1 = xmid + 1.
 CUCM_DEV void coder_encode_split(uint32_t prm, uint32_t xmid, uint32_t &x1, uint32_t &x2) {
   const uint32_t up = xmid + 1u;
   asm volatile(
@@ -327,6 +327,13 @@ CUCM_DEV uint32_t mix_weights(const int32_t w[8], const int32_t smp[8],
 CUCM_DEV uint32_t coder_mid(uint32_t x1, uint32_t x2, uint32_t pr) {
   return x1 + (uint32_t)(((uint64_t)(x2 - x1) * pr) >> 12);
 }
+// coder_mid with pr given as pr << 20 (pr < 2^12): floor(rng * pr / 2^12) is
+// the high word of rng * (pr << 20), one scalar multiply.
+CUCM_DEV uint32_t coder_mid20(uint32_t x1, uint32_t x2, uint32_t pr20) {
+  return x1 + (uint32_t)(((uint64_t)(x2 - x1) * pr20) >> 32);
+}
+// A unit bit's pr << 1 | y (as the encoder records it) as pr << 20 | y.
+CUCM_DEV uint32_t prm20(uint32_t prm) { return ((prm >> 1) << 20) | (prm & 1u); }
 
 // Zeroes (or copies the prelearned model into) the wave's model slot and
 // resets the LDS-resident parts of the model.
@@ -737,12 +744,13 @@ CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
 }
 
 // One step of the carry-less binary arithmetic coder (cucm_compress_v2.cu:
-// 841-868) for the packed prm = pr << 1 | y, in the uniform registers x1, x2;
-// the rare byte output goes through acc (4 bytes per store) to dst.
+// 841-868) for the packed prm = pr << 20 | y (prm20), in the uniform
+// registers x1, x2; the rare byte output goes through acc (4 bytes per store)
+// to dst.
 #define CUCM_CODE_BIT(prm_expr)                                           \
   do {                                                                    \
     const uint32_t prm_ = (prm_expr);                                     \
-    const uint32_t xmid_ = coder_mid(x1, x2, prm_ >> 1);                  \
+    const uint32_t xmid_ = coder_mid20(x1, x2, prm_ & 0xfff00000u);       \
     coder_encode_split(prm_, xmid_, x1, x2);                              \
     if (__builtin_expect(((x1 ^ x2) & 0xff000000u) == 0, 0)) {            \
       /* (rare) shift out the leading bytes x1 and x2 agree on */         \
@@ -802,7 +810,7 @@ CUCM_DEV void encode_task(const CodecArgs &a, const Task &tk, uint32_t t,
     }
     uint32_t pv[NJ];
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) pv[j] = (lane + 64u * j < Umax) ? M.rprm[lane + 64u * j] : 0u;
+    for (int j = 0; j < NJ; ++j) pv[j] = prm20((lane + 64u * j < Umax) ? M.rprm[lane + 64u * j] : 0u);
     const uint32_t nc = (uint32_t)min((uint64_t)U, coded_end - base);
     const uint32_t nwalk = nvalid >> 3;
 #pragma unroll
