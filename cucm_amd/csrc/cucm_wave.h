@@ -323,12 +323,8 @@ CUCM_DEV uint32_t mix_weights(const int32_t w[8], const int32_t smp[8],
 // Carry-less coder interval split (cucm_compress_v2.cu:855-857):
 // x1 + (rng >> 12) * pr + (((rng & 0xfff) * pr) >> 12) with rng = x2 - x1.
 // Writing rng = 4096a + b, that sum is a*pr + floor(b*pr / 4096) =
-// floor(rng * pr / 4096), one 32x32->64 product (rng * pr < 2^44).
-CUCM_DEV uint32_t coder_mid(uint32_t x1, uint32_t x2, uint32_t pr) {
-  return x1 + (uint32_t)(((uint64_t)(x2 - x1) * pr) >> 12);
-}
-// coder_mid with pr given as pr << 20 (pr < 2^12): floor(rng * pr / 2^12) is
-// the high word of rng * (pr << 20), one scalar multiply.
+// floor(rng * pr / 4096).  With pr given as pr20 = pr << 20 (pr < 2^12) that
+// is the high word of rng * pr20: one scalar multiply.
 CUCM_DEV uint32_t coder_mid20(uint32_t x1, uint32_t x2, uint32_t pr20) {
   return x1 + (uint32_t)(((uint64_t)(x2 - x1) * pr20) >> 32);
 }
@@ -1023,7 +1019,8 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
         const uint32_t q = (1u << i) - 1u + pre;
         const uint32_t v = path_value<NM>(pc, q);
         const uint32_t pr = v & 0xffffu, cx = v >> 16;
-        const uint32_t xmid = coder_mid(x1, x2, pr);
+        // (v << 20 is pr << 20: pr < 2^12 and cxt sits above bit 15)
+        const uint32_t xmid = coder_mid20(x1, x2, v << 20);
         const uint32_t y = coder_split(x, xmid, x1, x2);
         pre = (pre << 1) | y;
         if (__builtin_expect(((x1 ^ x2) & 0xff000000u) == 0, 0)) {
