@@ -431,7 +431,8 @@ CUCM_DEV void team_resolve_byte(const TeamMem &X, const LDS DevTables *T, uint32
 }
 
 // Wave 2: the statep walk of update_states' second part for byte b (its
-// bits y MSB first in ybyte) on stpn (lane k < 8 walks order k).  The walk restarts at every byte: the entries
+// bits y MSB first in ybyte) on stpn (lane k < 8 walks order k; callers run
+// it on those 8 lanes only).  The walk restarts at every byte: the entries
 // of its first two bits are read after the previous byte's writes, so it
 // never looks into a byte that is not resolved yet.
 #define CUCM_TWALK_BYTE(X, T, b, ybyte, Umax, lane)                           \
@@ -709,12 +710,12 @@ CUCM_DEV void team_walker(const Task &tk, const TeamMem &X, const LDS DevTables 
   for (uint32_t base = 0; base < L; base += U / 8) {
     const uint32_t nblk = min(U / 8, L - base);
     for (uint32_t b = 0; b <= nblk; ++b) {
-      if (b >= 2) CUCM_TWALK_BYTE(X, T, b - 2, X.M.ub[8 + b - 2], Umax, lane);
+      if (b >= 2 && lane < 8) CUCM_TWALK_BYTE(X, T, b - 2, X.M.ub[8 + b - 2], Umax, lane);
       clk.mark(7);
       team_sync_lds();
       clk.mark(3);
     }
-    CUCM_TWALK_BYTE(X, T, nblk - 1, X.M.ub[8 + nblk - 1], Umax, lane);
+    if (lane < 8) CUCM_TWALK_BYTE(X, T, nblk - 1, X.M.ub[8 + nblk - 1], Umax, lane);
     clk.mark(7);
     team_sync();
     team_refresh_sm(X, T, 2, lane);
@@ -1072,7 +1073,9 @@ CUCM_DEV void team_enc_walker(const Task &tk, const TeamMem &X, const LDS DevTab
   CUCM_WALK_STATE;
   for (uint32_t u = 0; u <= E.NU; ++u) {
     const uint32_t prev = u ? E.nvalid(u - 1) : 0u;
-    for (uint32_t b = 0; b < prev / 8; ++b) CUCM_TWALK_BYTE(X, T, b, X.ubw[b], Umax, lane);
+    // (only the 8 walking lanes run: the step's dummy store drops out)
+    if (lane < 8)
+      for (uint32_t b = 0; b < prev / 8; ++b) CUCM_TWALK_BYTE(X, T, b, X.ubw[b], Umax, lane);
     clk.mark(7);
     team_sync_lds();  // P1
     clk.mark(10);
