@@ -23,7 +23,8 @@ __device__ __forceinline__ void stage_tables(const DevTables *g, LDS DevTables *
 // is laid out for it, so every LDS offset is a compile-time constant.
 CUCM_HD uint32_t unit_capacity(uint32_t Umax) { return Umax <= 64 ? 64u : (Umax <= 128 ? 128u : 256u); }
 
-template <bool kDecode, int NJ, int D>
+// (kProf: the instantiation with phase clocks, launched when a.prof is set)
+template <bool kDecode, int NJ, int D, bool kProf>
 __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a) {
   constexpr uint32_t Umax = 64u * NJ;
   extern __shared__ __align__(16) uint8_t lds[];
@@ -44,9 +45,9 @@ __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a) {
     const Task tk = a.tasks[t];
     if (tk.flags & kTaskFrozen) continue;
     if constexpr (kDecode)
-      decode_task<NJ, D>(a, tk, t, M, T, slot, Umax, lane);
+      decode_task<NJ, D, kProf>(a, tk, t, M, T, slot, Umax, lane);
     else
-      encode_task<NJ>(a, tk, t, M, T, slot, Umax, lane);
+      encode_task<NJ, kProf>(a, tk, t, M, T, slot, Umax, lane);
   }
 }
 
@@ -348,14 +349,23 @@ static hipError_t launch_team(const CodecArgs &a, uint32_t Umax, uint32_t blocks
   }
 }
 
+template <bool kDecode, int NJ, int D, bool kProf>
+static hipError_t launch_one_prof(const CodecArgs &a, uint32_t blocks, uint32_t wpb, uint32_t lds,
+                                  hipStream_t stream) {
+  hipError_t e = hipFuncSetAttribute((const void *)codec_kernel<kDecode, NJ, D, kProf>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((codec_kernel<kDecode, NJ, D, kProf>), dim3(blocks), dim3(64 * wpb), lds,
+                     stream, a);
+  return hipGetLastError();
+}
+
 template <bool kDecode, int NJ, int D>
 static hipError_t launch_one(const CodecArgs &a, uint32_t Umax, uint32_t blocks, uint32_t wpb,
                              uint32_t lds, hipStream_t stream) {
-  hipError_t e = hipFuncSetAttribute((const void *)codec_kernel<kDecode, NJ, D>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((codec_kernel<kDecode, NJ, D>), dim3(blocks), dim3(64 * wpb), lds, stream, a);
-  return hipGetLastError();
+  (void)Umax;
+  return a.prof ? launch_one_prof<kDecode, NJ, D, true>(a, blocks, wpb, lds, stream)
+                : launch_one_prof<kDecode, NJ, D, false>(a, blocks, wpb, lds, stream);
 }
 
 template <int NJ>

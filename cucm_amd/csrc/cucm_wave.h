@@ -769,7 +769,7 @@ CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
 
 // One chunk.  Per unit: stage, hash and gather, predictions, the first part
 // of the deferred update, then the arithmetic coder and the statep walk.
-template <int NJ>
+template <int NJ, bool kProf>
 CUCM_DEV void encode_task(const CodecArgs &a, const Task &tk, uint32_t t,
                                             const WaveMem &M, const LDS DevTables *T,
                                             GLB uint8_t *slot, uint32_t Umax, uint32_t lane) {
@@ -778,7 +778,7 @@ CUCM_DEV void encode_task(const CodecArgs &a, const Task &tk, uint32_t t,
   const GLB uint8_t *src = (const GLB uint8_t *)(uintptr_t)tk.src;
   GLB uint8_t *dst = (GLB uint8_t *)(uintptr_t)tk.dst;
   const uint64_t cap = tk.aux;
-  PhaseClock clk(a.prof != nullptr);
+  PhaseClock clk(kProf && a.prof != nullptr);  // (clocks only when profiling)
   init_model(a, tk, M, slot, Umax, lane);
   clk.mark(0);
   const uint64_t vbits = 8 * L;
@@ -969,7 +969,7 @@ CUCM_DEV uint32_t path_value(const uint32_t pc[NM], uint32_t q) {
 // One chunk.  Each round decides D bits: the paths of the round are mixed,
 // the bits decoded by a scalar coder loop, then the next round's gathers are
 // issued before the current round's bits are recorded for the unit update.
-template <int NJ, int D>
+template <int NJ, int D, bool kProf>
 CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
                                             const WaveMem &M, const LDS DevTables *T,
                                             GLB uint8_t *slot, uint32_t Umax, uint32_t lane) {
@@ -981,7 +981,7 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
   const GLB uint8_t *pay = (const GLB uint8_t *)(uintptr_t)tk.src;
   const uint64_t plen = tk.aux;
   GLB uint8_t *dst = (GLB uint8_t *)(uintptr_t)tk.dst;
-  PhaseClock clk(a.prof != nullptr);
+  PhaseClock clk(kProf && a.prof != nullptr);  // (clocks only when profiling)
   init_model(a, tk, M, slot, Umax, lane);
   clk.mark(0);
   uint64_t winbase = 0;

@@ -92,7 +92,7 @@ static std::vector<uint8_t> encode_one(const uint8_t *chunk, uint32_t L, uint32_
   a.tasks = &tk;
   a.ntasks = 1;
   a.result = &result;
-  emu::run_wave([&](uint32_t lane) { encode_task<NJ>(a, tk, 0, lds.M, lds.T, slot, U, lane); });
+  emu::run_wave([&](uint32_t lane) { encode_task<NJ, true>(a, tk, 0, lds.M, lds.T, slot, U, lane); });
   if (result & kResultOverflow) {
     fprintf(stderr, "staging overflow\n");
     exit(3);
@@ -125,7 +125,7 @@ static void decode_one(const std::vector<uint8_t> &pay, uint8_t *out, uint32_t L
   a.ntasks = 1;
   a.spec_bits = D;
   a.result = &result;
-  emu::run_wave([&](uint32_t lane) { decode_task<NJ, D>(a, tk, 0, lds.M, lds.T, slot, U, lane); });
+  emu::run_wave([&](uint32_t lane) { decode_task<NJ, D, true>(a, tk, 0, lds.M, lds.T, slot, U, lane); });
   memcpy(out, dst, L);
   free(src);
   free(dst);
