@@ -707,6 +707,7 @@ CUCM_DEV void team_walker(const Task &tk, const TeamMem &X, const LDS DevTables 
                           uint32_t Umax, uint32_t lane, PhaseClock &clk) {
   const uint32_t U = tk.U, L = tk.L;
   CUCM_WALK_STATE;
+  (void)w_sv, (void)w_nblk, (void)w_blk;  // (the per-byte walk uses none of these)
   for (uint32_t base = 0; base < L; base += U / 8) {
     const uint32_t nblk = min(U / 8, L - base);
     for (uint32_t b = 0; b <= nblk; ++b) {
@@ -1071,6 +1072,7 @@ CUCM_DEV void team_enc_walker(const Task &tk, const TeamMem &X, const LDS DevTab
   const WaveMem &M = X.M;
   const EncUnits E(tk);
   CUCM_WALK_STATE;
+  (void)w_sv, (void)w_nblk, (void)w_blk;  // (the per-byte walk uses none of these)
   for (uint32_t u = 0; u <= E.NU; ++u) {
     const uint32_t prev = u ? E.nvalid(u - 1) : 0u;
     // (only the 8 walking lanes run: the step's dummy store drops out)
