@@ -26,7 +26,6 @@ EXPORTS = [
     "cucm_ctx_synchronize", "cucm_ctx_last_stats", "cucm_ctx_set_max_waves",
     "cucm_ctx_set_phase_profile", "cucm_ctx_phase_cycles",
     "cucm_serve_start", "cucm_serve_stop", "cucm_submit", "cucm_poll", "cucm_serve_launches",
-    "cucm_gather_probe", "cucm_copy_probe",
     "cucm_ctx_set_waves_per_block", "cucm_ctx_set_team", "cucm_container_head_len", "cucm_stream_compress_begin", "cucm_stream_compress_next",
     "cucm_stream_decompress_begin", "cucm_stream_decompress_next", "cucm_stream_end",
 ]
@@ -144,8 +143,6 @@ def lib():
         L.cucm_serve_stop.argtypes = [vp]
         L.cucm_submit.argtypes = [vp, P(Job), P(u64)]
         L.cucm_poll.argtypes = [vp, u64, ctypes.c_int, P(Result)]
-        L.cucm_gather_probe.argtypes = [vp, u64, u32, u32, u32, u32, u32, P(ctypes.c_double), P(u64)]
-        L.cucm_copy_probe.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         L.cucm_container_head_len.argtypes = [vp, u64, P(u64)]
         L.cucm_ctx_set_waves_per_block.argtypes = [vp, u32]
         L.cucm_ctx_set_team.argtypes = [vp, u32]
@@ -158,6 +155,26 @@ def lib():
         L.cucm_stream_end.restype = None
         _lib = L
     return _lib
+
+
+PROBE_PATH = os.path.join(HERE, "libcucm_probe.so")
+_probe = None
+
+
+def probe_lib():
+    """Loads libcucm_probe.so, the measurement kernels bench.py compares the
+    codec with (not part of the codec's C ABI)."""
+    global _probe
+    if _probe is None:
+        if not os.path.exists(PROBE_PATH):
+            raise CucmError(E_UNSUPPORTED, "libcucm_probe.so is not built (python -m cucm_amd.build)")
+        L = ctypes.CDLL(PROBE_PATH)
+        vp, u64, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32
+        P = ctypes.POINTER
+        L.cucm_probe_gather.argtypes = [vp, u64, u32, u32, u32, u32, u32, P(ctypes.c_double), P(u64)]
+        L.cucm_probe_copy.argtypes = [vp, vp, u64, P(ctypes.c_double)]
+        _probe = L
+    return _probe
 
 
 def _strerror(code):

@@ -2,6 +2,7 @@
 
   libcucm.so          HIP kernels (gfx950) + host engine + C ABI (include/cucm.h)
   libcucm_datagen.so  seeded synthetic input generators (host, OpenMP)
+  libcucm_probe.so    measurement kernels (gather ceiling, copy calibration)
   bin/cucm_compress, bin/cucm_decompress   drop-in command-line tools
 
 Run:  python -m cucm_amd.build   (or __graft_entry__.build()).
@@ -23,8 +24,12 @@ BIN = os.path.join(HERE, "bin")
 CLIS = ["cucm_compress", "cucm_decompress"]
 DATAGEN = os.path.join(HERE, "libcucm_datagen.so")
 
-HIP_SOURCES = ["cucm_kernels.hip", "cucm_engine.cpp", "cucm_host_model.cpp", "cucm_serve.cpp", "cucm_stream.cpp",
-               "gather_probe.hip"]
+HIP_SOURCES = ["cucm_kernels.hip", "cucm_engine.cpp", "cucm_host_model.cpp", "cucm_serve.cpp",
+               "cucm_stream.cpp"]
+# measurement kernels (bench.py's ceilings), a library of their own
+PROBE_DIR = os.path.join(HERE, "probe")
+PROBE_SOURCES = ["gather_probe.hip"]
+PROBE_LIB = os.path.join(HERE, "libcucm_probe.so")
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
@@ -50,8 +55,13 @@ def build(verbose=False):
         if src.endswith(".cpp"):
             cmd = [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         jobs.append((cmd, obj))
-    with concurrent.futures.ThreadPoolExecutor(max_workers=len(jobs) + 1) as ex:
-        futs = [ex.submit(_run, cmd) for cmd, _ in jobs]
+    probe_jobs = []
+    for src in PROBE_SOURCES:
+        obj = os.path.join(OBJ, "probe_" + os.path.splitext(src)[0] + ".o")
+        probe_jobs.append(([HIPCC] + HIP_FLAGS + ["-I" + PROBE_DIR, "-c", os.path.join(PROBE_DIR, src),
+                           "-o", obj], obj))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(jobs) + len(probe_jobs) + 1) as ex:
+        futs = [ex.submit(_run, cmd) for cmd, _ in jobs + probe_jobs]
         dg = ex.submit(_run, ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c11",
                               "-o", DATAGEN + ".tmp", os.path.join(CSRC, "datagen.c"), "-lm"])
         for f in futs:
@@ -61,6 +71,9 @@ def build(verbose=False):
         dg.result()
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", LIB + ".tmp"] + [o for _, o in jobs])
     os.replace(LIB + ".tmp", LIB)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", PROBE_LIB + ".tmp"] +
+         [o for _, o in probe_jobs])
+    os.replace(PROBE_LIB + ".tmp", PROBE_LIB)
     os.replace(DATAGEN + ".tmp", DATAGEN)
     # drop-in command-line tools: host C++ over the C ABI
     os.makedirs(BIN, exist_ok=True)

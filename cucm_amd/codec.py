@@ -221,13 +221,21 @@ class Context:
     def compress_batch(self, data, chunk_size, unit_bits=128, table_bits=25) -> bytes:
         addr, n, keep = _buf(data)
         C = (n + chunk_size - 1) // chunk_size
-        cap = 4 * C + C * (32 * chunk_size + 1)
-        cap = min(cap, n + n // 2 + 4096 * (C + 1))
-        out = np.empty(cap, dtype=np.uint8)
-        olen = ctypes.c_uint64(0)
-        check(lib().cucm_compress_batch(self.handle, addr, n, chunk_size, unit_bits, table_bits,
-                                        out.ctypes.data, cap, ctypes.byref(olen)), "compress_batch")
-        return out[: olen.value].tobytes()
+        strict = 4 * C + C * (32 * chunk_size + 1)
+        # the last chunk is coded up to the full chunk size (tail padding), so
+        # size from the padded length; a chunk that still overflows is
+        # retried once with the strict bound
+        padded = C * chunk_size
+        cap = min(strict, padded + padded // 2 + 4096 * (C + 1))
+        for attempt in (cap, strict):
+            out = np.empty(attempt, dtype=np.uint8)
+            olen = ctypes.c_uint64(0)
+            rc = lib().cucm_compress_batch(self.handle, addr, n, chunk_size, unit_bits, table_bits,
+                                           out.ctypes.data, attempt, ctypes.byref(olen))
+            if rc == -3 and attempt < strict:
+                continue
+            check(rc, "compress_batch")
+            return out[: olen.value].tobytes()
 
     def decompress_batch(self, blob, n, chunk_size, unit_bits=128, table_bits=25, spec_bits=4) -> bytes:
         addr, bl, keep = _buf(blob)

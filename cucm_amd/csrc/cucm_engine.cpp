@@ -623,7 +623,11 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   InitModels im;
   std::vector<ParseJob> pj;
   std::vector<uint32_t> pj_of;
-  uint32_t ntasks = 0, nfrozen = 0, Umax = 32, tbmax = 8;
+  // Chunk counts come from untrusted headers: each chunk needs at least a
+  // 4-byte end offset and one payload byte, so a count above (len - seg) / 5
+  // is corrupt; totals are kept in 64 bits and refused above 2^32 - 1.
+  uint64_t ntasks = 0, nfrozen = 0;
+  uint32_t Umax = 32, tbmax = 8;
   for (uint32_t j = 0; j < n; ++j) {
     DecJob &J = jobs[j];
     J.status = CUCM_OK;
@@ -644,13 +648,19 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
         im.hostside.push_back(*J.ext_model);
         model = (int)im.hostside.size() - 1;
       }
+      const uint64_t C = (J.body + J.S - 1) / J.S;
+      if (C > J.len / 5 || ntasks + C > 0xffffffffull) {
+        J.status = CUCM_E_CORRUPT;
+        J.out_len = 0;
+        continue;
+      }
       ParseJob p{};
       p.src = (uint64_t)(uintptr_t)J.din;
       p.len = J.len;
       p.dst = (uint64_t)(uintptr_t)J.dout;
       p.body = J.body;
-      p.first_task = ntasks;
-      p.ntasks = (uint32_t)((J.body + J.S - 1) / J.S);
+      p.first_task = (uint32_t)ntasks;
+      p.ntasks = (uint32_t)C;
       p.seg = 0;
       p.S = J.S;
       p.U = J.U;
@@ -697,6 +707,11 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
       J.status = CUCM_E_UNSUPPORTED;
       continue;
     }
+    const uint64_t C = body ? (body + S - 1) / S : 0;
+    if (C > (J.len - kHeader - segb) / 5 || ntasks + C > 0xffffffffull) {
+      J.status = CUCM_E_CORRUPT;
+      continue;
+    }
     int model = -1;
     if (P > 0) {
       std::vector<uint8_t> segbuf(segb), out(P);
@@ -727,8 +742,8 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
     p.len = J.len;
     p.dst = (uint64_t)(uintptr_t)(J.dout + P);
     p.body = body;
-    p.first_task = ntasks;
-    p.ntasks = (uint32_t)((body + S - 1) / S);
+    p.first_task = (uint32_t)ntasks;
+    p.ntasks = (uint32_t)C;
     p.seg = (uint32_t)(kHeader + segb);
     p.S = (uint32_t)S;
     p.U = U;
@@ -775,7 +790,7 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   st.waves = std::max<uint64_t>(st.waves, sh.waves() + fz.waves());
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
-  a.ntasks = ntasks;
+  a.ntasks = (uint32_t)ntasks;
   a.spec_bits = spec;
   a.counter = (uint32_t *)c->counter.p;
   a.pool = (uint8_t *)c->pool.p;

@@ -237,8 +237,14 @@ int cucm_poll(cucm_ctx *ctx, uint64_t job_id, int wait, cucm_result *res) {
     if (!wait) return CUCM_E_NOTREADY;
     // look the job up again after every wake-up: submissions made while
     // this thread waited may have rehashed the table
-    s->cv_done.wait(lk, [&] { return s->jobs.find(job_id)->second.done; });
+    // this thread waited may have rehashed the table.  Each id has one
+    // waiter; if another thread reported the job meanwhile it is gone.
+    s->cv_done.wait(lk, [&] {
+      auto f = s->jobs.find(job_id);
+      return f == s->jobs.end() || f->second.done;
+    });
     it = s->jobs.find(job_id);
+    if (it == s->jobs.end()) return CUCM_E_PARAM;
   }
   const SJob &j = it->second;
   res->status = j.status;

@@ -1,5 +1,7 @@
-// gather_probe.hip -- random one-byte gather microbenchmark (the ceiling the
-// codec's state gathers are compared against).
+// gather_probe.hip -- measurement kernels, built into libcucm_probe.so (not
+// part of the codec library or its C ABI): a random one-byte gather
+// microbenchmark (the ceiling the codec's state gathers are compared
+// against) and a streaming copy (HBM counter calibration).
 //
 // Two modes over a table of `bytes` bytes:
 //   latency:    each lane follows a dependent chain of random one-byte reads
@@ -10,6 +12,8 @@
 // Results (per launch): total touches and elapsed shader clocks per wave.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "cucm_probe.h"
 
 namespace {
 
@@ -23,7 +27,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 }
 
 __global__ void gather_kernel(uint8_t *table, uint64_t bytes, uint32_t steps, uint32_t width,
-                              uint32_t write_back, uint64_t *clocks, uint64_t *sink) {
+                              uint32_t write_back, uint64_t *clocks, uint32_t *sink) {
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t h = mix64(gid + 1);
   uint32_t acc = 0;
@@ -45,7 +49,8 @@ __global__ void gather_kernel(uint8_t *table, uint64_t bytes, uint32_t steps, ui
   }
   const uint64_t t1 = clock64();
   if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long *)clocks, (unsigned long long)(t1 - t0));
-  if (acc == 0xdeadbeefu) sink[gid] = acc;
+  // keeps the reads live; sink is one word, so every lane adds to it
+  if (acc == 0xdeadbeefu) atomicAdd(sink, 1u);
 }
 
 __global__ void copy_kernel(const uint4 *src, uint4 *dst, uint64_t n16) {
@@ -58,7 +63,8 @@ __global__ void copy_kernel(const uint4 *src, uint4 *dst, uint64_t n16) {
 
 // Plain streaming copy of `bytes` (a multiple of 16): the reference kernel
 // whose FETCH_SIZE / WRITE_SIZE readings calibrate the HBM traffic counters.
-extern "C" int cucm_copy_probe(void *dst, const void *src, uint64_t bytes, double *ms) {
+extern "C" int cucm_probe_copy(void *dst, const void *src, uint64_t bytes, double *ms) {
+  if (!dst || !src || !ms || bytes % 16) return -1;
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
@@ -75,9 +81,12 @@ extern "C" int cucm_copy_probe(void *dst, const void *src, uint64_t bytes, doubl
   return err == hipSuccess ? 0 : -6;
 }
 
-extern "C" int cucm_gather_probe(void *table, uint64_t bytes, uint32_t blocks, uint32_t threads,
+extern "C" int cucm_probe_gather(void *table, uint64_t bytes, uint32_t blocks, uint32_t threads,
                                  uint32_t steps, uint32_t width, uint32_t write_back,
                                  double *ms, uint64_t *wave_clocks) {
+  if (!table || bytes == 0 || (bytes & (bytes - 1)) || width == 0 || width > 8 || !ms ||
+      !wave_clocks)
+    return -1;
   uint64_t *d = nullptr;
   if (hipMalloc(&d, 16) != hipSuccess) return -4;
   (void)hipMemset(d, 0, 16);
@@ -86,7 +95,7 @@ extern "C" int cucm_gather_probe(void *table, uint64_t bytes, uint32_t blocks, u
   (void)hipEventCreate(&e1);
   (void)hipEventRecord(e0, 0);
   hipLaunchKernelGGL(gather_kernel, dim3(blocks), dim3(threads), 0, 0, (uint8_t *)table, bytes,
-                     steps, width, write_back, d, d + 1);
+                     steps, width, write_back, d, (uint32_t *)(d + 1));
   (void)hipEventRecord(e1, 0);
   hipError_t err = hipEventSynchronize(e1);
   float f = 0;

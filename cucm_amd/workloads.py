@@ -271,7 +271,7 @@ def gather_workload(args):
     the stream workload's live model (resident waves x 7 order tables of
     2^table_bits bytes): the ceiling for the codec's state gathers."""
     import ctypes
-    from ._native import lib
+    from ._native import probe_lib
     tb = getattr(args, "gather_tb", 22)
     waves = getattr(args, "gather_waves", 256)
     footprint = waves * 7 * (1 << tb)
@@ -284,7 +284,7 @@ def gather_workload(args):
                                                 ("read_throughput", 4096, 256, 200, 8, 0)]:
         ms = ctypes.c_double()
         clk = ctypes.c_uint64()
-        rc = lib().cucm_gather_probe(buf.addr, bytes_, blocks, thr, steps, width, wb,
+        rc = probe_lib().cucm_probe_gather(buf.addr, bytes_, blocks, thr, steps, width, wb,
                                      ctypes.byref(ms), ctypes.byref(clk))
         assert rc == 0
         nw = blocks * thr // 64
@@ -297,7 +297,7 @@ def gather_workload(args):
 def copy_workload(args):
     """A plain 16-byte streaming copy of --mib MiB (HBM counter calibration)."""
     import ctypes
-    from ._native import lib
+    from ._native import probe_lib
     n = (args.mib or 256) * MiB
     ctx = Context()
     a = ctx.alloc(n)
@@ -305,6 +305,6 @@ def copy_workload(args):
     out = []
     for _ in range(3):
         ms = ctypes.c_double()
-        assert lib().cucm_copy_probe(b.addr, a.addr, n, ctypes.byref(ms)) == 0
+        assert probe_lib().cucm_probe_copy(b.addr, a.addr, n, ctypes.byref(ms)) == 0
         out.append(ms.value)
     return {"metric": "copy kernel", "bytes": n, "ms": out, "GBps": 2 * n / (min(out) / 1e3) / 1e9}

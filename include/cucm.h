@@ -221,19 +221,6 @@ int cucm_ctx_set_waves_per_block(cucm_ctx *ctx, uint32_t waves);
  * at least two full passes of resident teams. */
 int cucm_ctx_set_team(cucm_ctx *ctx, uint32_t mode);
 
-/* ---- measurement probes (device 0 / current device) ------------------- */
-/* Random one-byte gather microbenchmark over `bytes` (a power of two) of
- * device memory `table`: blocks x threads lanes each do `steps` rounds of
- * `width` independent random reads (and, if write_back, writes of the same
- * bytes).  The measured touch rate is the ceiling the codec's state gathers
- * are compared against.  No counterpart in the reference. */
-int cucm_gather_probe(void *table, uint64_t bytes, uint32_t blocks, uint32_t threads,
-                      uint32_t steps, uint32_t width, uint32_t write_back, double *ms,
-                      uint64_t *wave_clocks);
-/* Streaming device copy of `bytes` (multiple of 16), used to calibrate the
- * HBM traffic counters.  No counterpart in the reference. */
-int cucm_copy_probe(void *dst, const void *src, uint64_t bytes, double *ms);
-
 /* ---- serving: continuous batching ------------------------------------ */
 /* No counterpart in the reference (one file per process run there).
  * Requests of mixed kinds and sizes are queued as they arrive; each of

@@ -106,6 +106,17 @@ def test_batch_records(ctx, oracle):
     assert ctx.decompress_batch(blob, len(data), 8192, 64, 16) == data
 
 
+@pytest.mark.parametrize("n,S", [(100, 1 << 20), (5000, 1 << 20), (100, 65536), (70000, 65536)])
+def test_batch_records_much_smaller_than_chunk(ctx, oracle, n, S):
+    """The last chunk is tail-padded to the full chunk size, so a blob of a
+    few bytes can code to most of a chunk; the binding sizes its buffer from
+    the padded length (and retries with the strict bound)."""
+    data = datagen.markov2(n, 28).tobytes()
+    blob = ctx.compress_batch(data, S, 128, 12)
+    assert blob == oracle.encode_batch(data, S, 128, oracle.Model(12))
+    assert ctx.decompress_batch(blob, n, S, 128, 12) == data
+
+
 def test_no_tail_pad_flag(ctx, oracle):
     data = datagen.markov2(5000, 23).tobytes()
     a = ctx.compress(data, make_params(4096, 128, 100, 0, 18, tail_pad=False))
@@ -142,6 +153,44 @@ def test_corrupt_and_truncated_containers_are_reported(ctx):
     assert rc == 0 and st1[0] == -3
     # the context is still usable
     assert ctx.decompress(blob) == data
+
+
+def test_crafted_chunk_counts_are_corrupt_not_sized(ctx):
+    """A 21-byte header claiming N = 2^31 bytes in 1-byte chunks has no room
+    for its 2^31 end offsets; two of them in one call once wrapped the 32-bit
+    task count.  Both must come back CUCM_E_CORRUPT while the valid container
+    beside them decodes (each chunk needs a 4-byte offset and a payload
+    byte, so a count above (len - header) / 5 is refused before sizing)."""
+    import ctypes
+    import struct
+    from cucm_amd._native import lib
+    data = datagen.markov2(20000, 27).tobytes()
+    good = ctx.compress(data, make_params(4096, 128, 100, 0, 18))
+    crafted = (struct.pack(">IHHIII", 1 << 31, 100, 128 | 18, 1, 0, 0) + b"\x00" +
+               struct.pack("<I", 0) + b"\x00" * 11)
+    bad = [crafted, crafted, good]
+    n = len(bad)
+    outs = [np.empty(len(data), dtype=np.uint8) for _ in bad]
+    keep = [ctypes.create_string_buffer(b, len(b)) for b in bad]
+    ins = (ctypes.c_void_p * n)(*[ctypes.addressof(k) for k in keep])
+    lens = (ctypes.c_uint64 * n)(*[len(b) for b in bad])
+    outp = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
+    caps = (ctypes.c_uint64 * n)(*([1 << 31, 1 << 31, len(data)]))
+    olen = (ctypes.c_uint64 * n)()
+    st = (ctypes.c_int32 * n)()
+    # device-resident decode: the crafted containers' capacity is not backed
+    # by memory, so only a refusal before any task is built keeps this safe
+    d_in = [ctx.alloc(len(b)) for b in bad]
+    for d, b in zip(d_in, bad):
+        d.upload(b)
+    d_out = ctx.alloc(len(data))
+    dins = (ctypes.c_void_p * n)(*[d.addr for d in d_in])
+    douts = (ctypes.c_void_p * n)(*[d_out.addr] * n)
+    rc = lib().cucm_decompress_many_device(ctx.handle, n, dins, lens, douts, caps, olen, st, 4)
+    assert rc == 0
+    assert list(st) == [-2, -2, 0]
+    assert olen[2] == len(data)
+    assert d_out.download(len(data)) == data
 
 
 def test_bad_params_rejected(ctx):
