@@ -28,10 +28,14 @@
  *   FNV hash           textbook FNV-1a-32 vectors
  *   StateMap/Mixer     closed-form first-update / zero-weight known answers
  *   coder              first-step known answer + invertibility
- *   unit schedule, tail padding, framing:  clean vs literal encoders, clean
- *                      vs literal decoders, round trips.  These are pinned
- *                      only by cross-checking two independently structured
- *                      restatements ("parity unpinned" by any reference run).
+ *   unit schedule, tail padding, zeroed staging:  closed forms of the zero
+ *                      model (one-unit chunks code their complement; zeroed
+ *                      slots step x1 += 1), a hand-derived two-unit known
+ *                      answer, the stale-slot rule on per-bit traces, and a
+ *                      restatement of the schedule from the pinned primitives
+ *                      (tests/test_oracle_schedule.py)
+ *   framing            big-endian offsets/lengths checked field by field;
+ *                      clean vs literal encoders and decoders; round trips
  */
 #ifndef CUCM_ORACLE_H
 #define CUCM_ORACLE_H

@@ -55,12 +55,19 @@ def test_clean_and_literal_encoders_agree(N, S, U, tb, kind):
 @pytest.mark.parametrize("N,S,U,tb,kind", [c for c in CASES if c[1] >= 8])
 @pytest.mark.parametrize("m", [0, 1, 3])
 def test_reference_decoder_restatement_accepts_stream(N, S, U, tb, kind, m):
+    """Every case's chunk payloads are at least 4 bytes, so the reference
+    decoder's 4-byte x initialisation stays inside each payload (flags 0);
+    its restatement must then return the input for every speculation depth."""
     data = make(kind, N, N + S)
     model = O.Model(tb)
     blob = O.encode_batch(data, S, U, model)
+    C = (N + S - 1) // S
+    ends = [int.from_bytes(blob[4 * c:4 * c + 4], "big") for c in range(C)]
+    sizes = [e - p for e, p in zip(ends, [4 * C - 1] + ends[:-1])]
+    assert min(sizes) >= 4
     out, flags = O.decode_batch(blob, N, S, U, model, literal_m=m)
-    if flags == 0:
-        assert out == data
+    assert flags == 0
+    assert out == data
 
 
 def test_chunks_below_8_bytes_round_trip():
