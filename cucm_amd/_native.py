@@ -28,6 +28,7 @@ EXPORTS = [
     "cucm_serve_start", "cucm_serve_stop", "cucm_submit", "cucm_poll", "cucm_serve_launches",
     "cucm_ctx_set_waves_per_block", "cucm_ctx_set_team", "cucm_container_head_len", "cucm_stream_compress_begin", "cucm_stream_compress_next",
     "cucm_stream_decompress_begin", "cucm_stream_decompress_next", "cucm_stream_end",
+    "cucm_compress_device_noalloc",
 ]
 
 
@@ -151,6 +152,7 @@ def lib():
         L.cucm_stream_decompress_begin.argtypes = [vp, vp, u64, u32, P(u64), P(Params), vp, u64,
                                                    P(vp)]
         L.cucm_stream_decompress_next.argtypes = [vp, vp, u64, vp, u64, P(u64)]
+        L.cucm_compress_device_noalloc.argtypes = [vp, vp, u64, P(Params), vp, u64, P(u64)]
         L.cucm_stream_end.argtypes = [vp]
         L.cucm_stream_end.restype = None
         _lib = L

@@ -25,7 +25,7 @@ CLIS = ["cucm_compress", "cucm_decompress"]
 DATAGEN = os.path.join(HERE, "libcucm_datagen.so")
 
 HIP_SOURCES = ["cucm_kernels.hip", "cucm_engine.cpp", "cucm_host_model.cpp", "cucm_serve.cpp",
-               "cucm_stream.cpp"]
+               "cucm_stream.cpp", "cucm_noalloc.hip"]
 # measurement kernels (bench.py's ceilings), a library of their own
 PROBE_DIR = os.path.join(HERE, "probe")
 PROBE_SOURCES = ["gather_probe.hip"]

@@ -262,6 +262,14 @@ class Context:
             check(lib().cucm_compress_many_device(self.handle, n, a, l, ps, o, c, olen, st), "compress_device")
         return list(olen), list(st)
 
+    def compress_device_noalloc(self, d_in, n, params, d_out, cap):
+        """One device-resident input through the allocation-free entry point;
+        returns the container length."""
+        olen = ctypes.c_uint64(0)
+        check(lib().cucm_compress_device_noalloc(self.handle, d_in, n, ctypes.byref(params), d_out,
+                                                 cap, ctypes.byref(olen)), "compress_noalloc")
+        return olen.value
+
     def decompress_many_device(self, ins, lens, outs, caps, spec_bits=4, plain=False):
         n = len(ins)
         a = (ctypes.c_void_p * n)(*ins)

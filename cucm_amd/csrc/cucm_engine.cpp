@@ -50,7 +50,6 @@ inline uint32_t get_be32(const uint8_t *p) {
 // Staging capacity of one chunk.  Real data stays below ~1.01 S; a chunk
 // that overflows is re-run with the strict bound (the carry-less coder can
 // emit at most 4 bytes per coded bit).
-inline uint64_t staging_cap(uint64_t S) { return align_up(S + S / 2 + 256, 16); }
 inline uint64_t strict_cap(uint64_t S) { return align_up(32 * S + 16, 16); }
 
 }  // namespace
@@ -165,6 +164,9 @@ int ctx_init(cucm_ctx *c, int device, uint64_t budget) {
   size_t freeb = 0, total = 0;
   HIP_TRY(hipMemGetInfo(&freeb, &total));
   c->budget = budget ? budget : (uint64_t)(freeb * 0.9);
+  // the allocation-free entry point's scratch address on this device; if
+  // the lookup fails only that entry point reports CUCM_E_DEVICE
+  (void)noalloc_setup(device);
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
   CU_TRY(c->tabs.reserve(sizeof(DevTables)));
@@ -198,17 +200,6 @@ int validate_params(const cucm_params &p, uint64_t n) {
 
 namespace {
 
-// Waves and launch shape for a persistent launch of `ntasks` chunks: either
-// single-wave chunks (wpb waves per workgroup sharing the tables, one model
-// slot per wave) or teams (one chunk and one model slot per workgroup of
-// kTeamWaves waves).
-struct Shape {
-  uint32_t blocks = 0, wpb = 0;
-  bool team = false;
-  uint32_t waves() const { return blocks * wpb; }
-  uint32_t slots() const { return team ? blocks : blocks * wpb; }
-};
-
 // Team workgroups the launch can hold at once (0 if a team does not fit).
 uint64_t team_capacity(const cucm_ctx *c, uint32_t Umax, uint32_t spec, bool dec) {
   const uint32_t lds = team_codec_lds_bytes(Umax, spec, dec);
@@ -228,6 +219,8 @@ bool use_team(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, 
   if (cap == 0) return false;
   return c->team_mode == 2 || ntasks <= cap || ntasks >= 2 * cap;
 }
+
+}  // namespace
 
 Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, bool dec,
                  uint64_t slot_bytes, uint64_t mem_left) {
@@ -289,7 +282,7 @@ Shape frozen_shape(const cucm_ctx *c, uint64_t ntasks) {
 // With `timed`, events 0 and 1 bracket both launches.
 int launch_pass(cucm_ctx *c, const CodecArgs &a, bool decode, uint32_t Umax, const Shape &sh,
                 const Shape &fz, bool timed, cucm_stats &st) {
-  HIP_TRY(hipMemsetAsync(c->counter.p, 0, 8, c->stream));
+  HIP_TRY(hipMemsetAsync(a.counter, 0, 8, c->stream));
   if (timed) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   if (sh.blocks) {
     HIP_TRY(launch_codec(decode, a, Umax, sh.blocks, sh.wpb, sh.team, c->stream));
@@ -304,6 +297,8 @@ int launch_pass(cucm_ctx *c, const CodecArgs &a, bool decode, uint32_t Umax, con
   if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   return CUCM_OK;
 }
+
+namespace {
 
 struct InitModels {
   std::vector<DevInitModel> hostside;

@@ -98,7 +98,36 @@ struct DecJob {
   const cucm::DevInitModel *ext_model = nullptr;
 };
 
+// Waves and launch shape for a persistent launch of `ntasks` chunks: either
+// single-wave chunks (wpb waves per workgroup sharing the tables, one model
+// slot per wave) or teams (one chunk and one model slot per workgroup of
+// kTeamWaves waves).
+struct Shape {
+  uint32_t blocks = 0, wpb = 0;
+  bool team = false;
+  uint32_t waves() const { return blocks * wpb; }
+  uint32_t slots() const { return team ? blocks : blocks * wpb; }
+};
+// Shape of the unit codec's launch within mem_left bytes of model slots of
+// slot_bytes each (no slot limit when slot_bytes is 0).
+Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, bool dec,
+                 uint64_t slot_bytes, uint64_t mem_left);
+// Shape of the frozen-model codec's launch (no model slots).
+Shape frozen_shape(const cucm_ctx *c, uint64_t ntasks);
+// One codec pass: the unit codec (sh) and the frozen codec (fz) back to back
+// on the context's stream, each pulling from its own queue head; the two
+// head words at a.counter must be zero.  With `timed`, events 0 and 1
+// bracket both launches.
+int launch_pass(cucm_ctx *c, const cucm::CodecArgs &a, bool decode, uint32_t Umax,
+                const Shape &sh, const Shape &fz, bool timed, cucm_stats &st);
+// Staging capacity of one chunk's payload before assembly (real data stays
+// below ~1.01 S; an overflow is flagged in the chunk's result word).
+inline uint64_t staging_cap(uint64_t S) { return (S + S / 2 + 256 + 15) / 16 * 16; }
+
 int ctx_init(cucm_ctx *c, int device, uint64_t budget);
+// Looks up the allocation-free entry point's device scratch on `device`
+// (once per device; the device must be current).
+int noalloc_setup(int device);
 int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st);
 int encode_assemble(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st);
 int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats &st);

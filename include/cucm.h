@@ -138,6 +138,24 @@ int cucm_decompress_plain_device(cucm_ctx *ctx, uint32_t n, const uint8_t *const
                                  const uint64_t out_cap[], uint64_t out_len[],
                                  uint32_t spec_bits);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (serving loops with a fixed footprint, code that must not call the
+ * allocator).  Codes one device-resident input into a device-resident
+ * container exactly as cucm_compress_many_device does, taking all its
+ * scratch (tasks, results, payload staging, offsets, model slots) from one
+ * 64 MiB device array that each GPU holds from library load until the
+ * process ends; it calls no hipMalloc/hipFree and no allocating routine.
+ * Limits: prelearn_size must be 0 (CUCM_E_UNSUPPORTED otherwise); the chunk
+ * staging (1.5 S + 256 B per chunk) plus one model slot (7 x 2^table_bits B,
+ * none for the frozen model) must fit the array (CUCM_E_BUDGET otherwise --
+ * e.g. up to ~24 MiB of input at table_bits 18, ~1 MiB at 25); a chunk whose
+ * payload exceeds its staging (adversarial input only) gives
+ * CUCM_E_UNSUPPORTED.  On CUCM_E_SMALL *out_len holds the container size.
+ * Calls on one GPU share the array and are serialised in this process. */
+int cucm_compress_device_noalloc(cucm_ctx *ctx, const uint8_t *d_in, uint64_t in_len,
+                                 const cucm_params *p, uint8_t *d_out, uint64_t out_cap,
+                                 uint64_t *out_len);
+
 /* ---- streaming container I/O ----------------------------------------- */
 /* The reference's compress_file / decompress (cucm_compress_v2.cu:1151-1250,
  * cucm_decompress_v2.cu:1184-1248) hold the whole file in host memory; a
