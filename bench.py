@@ -1,28 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X CuCM codec (one JSON line on stdout).
 
-Default workload (the metric's single-GPU configuration): a 256 MiB mixed
-synthetic stream (60% order-3 Markov text, 25% LZ copies, 15% random 4 KiB
-blocks; seed 2) cut into 1 MiB chunks, reference model with 2^22 slots per
-order table, U = 128, P = 0, 100 chunks per batch record.
+Default workload (the metric's batched configuration): 10,000 independent
+files with lognormal sizes (median 64 KiB, sigma 1.5, clipped to [512 B,
+64 MiB]), file i from the order-2 Markov source over 64 symbols seeded
+1000 + i; per file, chunks of at most 128 KiB and 2^18 slots per order table
+(the autotuner's ratio-constrained choice, kept in cucm_amd/batch_settings.json
+with the launch settings measured on the full batch), U = 128, P = 0.
 
-One step = compress the device-resident input into a device-resident
-container, then decompress it back (both through the C ABI, one persistent
-launch each).  `value` is input MB (1e6 B) per second of step time; the
-compress and decompress rates are reported beside it.  Outputs are checked
-after the timed region: the round trip must be lossless, and the payloads of
-a sample of chunks must equal the oracle's (the CPU restatement of the
-reference), which is also timed as the CPU baseline.
+One step = compress every file (device-resident inputs and containers) in one
+persistent launch, then decompress every file in another, through the C ABI.
+`value` is input MB (1e6 B) per second of step time; the compress and
+decompress rates, the plain path (one launch per file, as the reference
+launches per batch, on a seeded 40-file sample) and the ratio are reported
+beside it.  After the timed region the round trip must be lossless; the CPU
+baseline (the oracle, files as tasks on every host thread and on one, warm-up
+then the median of 3) is timed on size-stratified samples of the same files,
+whose containers must equal the GPU's.
 
-Other workloads: --workload structured (1 GiB structured binary, 256 KiB
-chunks, 2^22 slots), --workload small (1 MiB order-2 Markov, 16 KiB chunks,
-2^18 slots), --workload batch (10,000 lognormal files, batched vs plain
-launch), --workload serve (Poisson request mix, p50/p99), --workload gather
-(random-touch ceiling), --workload copy (counter calibration).
-
---frozen runs a single-stream workload with the frozen model (the initial
-model, trained on the first --prelearn-kib KiB of the input, codes every
-chunk without updates; CUCM_FLAG_FROZEN).
+Other workloads: --workload stream (256 MiB mixed stream, 1 MiB chunks, 2^22
+slots), structured (1 GiB structured binary, 256 KiB chunks, 2^22 slots),
+small (1 MiB order-2 Markov, 16 KiB chunks, 2^18 slots), serve (Poisson
+request mix, p50/p99), gather (random-touch ceiling), copy (counter
+calibration).  --frozen runs a single-stream workload with the frozen model.
 """
 import argparse
 import json
@@ -295,7 +295,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="stream",
+    ap.add_argument("--workload", default="batch",
                     choices=["stream", "structured", "small", "batch", "serve", "gather", "copy"])
     ap.add_argument("--mib", type=int, default=0, help="input size (0 = the workload's own)")
     ap.add_argument("--spec", type=int, default=4)
@@ -306,15 +306,17 @@ def main():
                          "--frozen, else 0)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-chunks", type=int, default=0)
+    ap.add_argument("--cpu-chunks", type=int, default=0, help="stream workloads: CPU sample chunks")
     ap.add_argument("--files", type=int, default=10000, help="batch workload: number of files")
-    ap.add_argument("--chunk-kib", type=int, default=256, help="batch workload: largest chunk")
     ap.add_argument("--plain-files", type=int, default=40,
-                    help="batch workload: files in the one-launch-per-file sample")
-    ap.add_argument("--autotune", dest="autotune", action="store_true", default=True,
-                    help="batch workload: tune chunk size, table bits, speculation and waves per "
-                         "workgroup on a sample first (default)")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+                    help="batch workload: files in the one-launch-per-file sample (0: skip)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch workload: tune chunk size, table bits and each direction's launch "
+                         "settings on a sample first instead of using cucm_amd/batch_settings.json")
+    ap.add_argument("--save-settings", action="store_true",
+                    help="with --autotune: write the result to cucm_amd/batch_settings.json")
+    ap.add_argument("--cpu-mib", type=int, default=12,
+                    help="batch workload: MiB of files in the all-thread CPU baseline sample")
     ap.add_argument("--requests", type=int, default=240, help="serve workload: requests per run")
     ap.add_argument("--lanes", type=int, default=0, help="serve workload: concurrent lanes (0 = 3)")
     args = ap.parse_args()

@@ -1,18 +1,20 @@
 """Benchmark workloads beyond the default stream (used by bench.py).
 
-batch_workload: 10,000 independent files, lognormal sizes (median 64 KiB,
-sigma 1.5, clipped to [512 B, 64 MiB]), file i generated by the order-2
-Markov source seeded 1000 + i; 2^18 slots per order table and chunks of at
-most 256 KiB, or the settings the autotuner picks on a sample (--autotune).  All files are
-encoded, then decoded, in one persistent launch each (device-resident
+batch_workload (bench.py's default): 10,000 independent files, lognormal
+sizes (median 64 KiB, sigma 1.5, clipped to [512 B, 64 MiB]), file i
+generated by the order-2 Markov source seeded 1000 + i, with the settings of
+cucm_amd/batch_settings.json (chunks of at most 128 KiB, 2^18 slots, the
+launch settings measured on the full batch) or freshly autotuned.  All files
+are encoded, then decoded, in one persistent launch each (device-resident
 buffers); the plain path issues one launch per file, as the reference does
 per batch, and is timed on a seeded random sample of the same files because
-each single-file launch runs alone on one wave.
+each single-file launch runs alone.
 
 serve_workload: Poisson arrivals of mixed compress/decompress requests
 through the continuous-batching serving loop (cucm_serve_*), p50/p99 per
 load level beside the one-request-at-a-time plain path.
 """
+import json
 import os
 import sys
 import time
@@ -35,24 +37,55 @@ def pick_chunk_size(n, s_max):
     return int(max(1, min(n, s_max)))
 
 
-def batch_workload(args):
-    count = getattr(args, "files", 10000) or 10000
-    s_max = getattr(args, "chunk_kib", 256) * 1024
-    tb = 18
-    spec = args.spec
-    t0 = time.time()
+METRIC = "compress/decompress MB/s on 1 GPU at identical ratio; batched vs plain; p99"
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES_PER_INPUT_BYTE = 128.0
+BATCH_TRAFFIC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                             "profiles", "batch_traffic.json")
+
+
+def batch_files(count=10000):
+    """The uneven-size batch: count files, sizes lognormal (median 64 KiB,
+    sigma 1.5) clipped to [512 B, 64 MiB] (seed 3), file i from the order-2
+    Markov source seeded 1000 + i.  Returns (sizes, arena, offsets)."""
     sizes = datagen.lognormal_sizes(count, 65536, 1.5, 512, 64 * MiB, 3)
     arena, offs = datagen.markov2_files(sizes, seed0=1000)
+    return sizes, arena, offs
+
+
+def batch_kernel_name(launch, direction):
+    if direction == "decoder":
+        return "decode_team_kernel" if launch["team"] else "codec_kernel<decode>"
+    return "encode_team_kernel" if launch["team"] else "codec_kernel<encode>"
+
+
+def batch_workload(args):
+    """The uneven-size batch, every file compressed in one persistent launch
+    and decompressed in another per step (device-resident buffers), with the
+    stream and launch settings of cucm_amd/batch_settings.json (or freshly
+    autotuned with --autotune).  Beside it: the plain path (one launch per
+    file, as the reference issues one launch sequence per batch) on a seeded
+    sample of the same files, the CPU baseline (the oracle, files as tasks,
+    all host cores and one), and the roofline of the dominant kernel."""
+    from .autotune import apply_launch, autotune_batch, load_settings, save_settings
+    count = getattr(args, "files", 10000) or 10000
+    t0 = time.time()
+    sizes, arena, offs = batch_files(count)
     total = int(sizes.sum())
     log("generated %d files, %.1f MiB in %.1fs" % (count, total / MiB, time.time() - t0))
     ctx = Context()
-    tuned = None
     if getattr(args, "autotune", False):
-        from .autotune import autotune_batch
         tuned = autotune_batch(ctx, arena, offs, sizes, log=log)
-        s_max, tb, spec = tuned["chunk_max"], tuned["table_bits"], tuned["spec_bits"]
-        ctx.set_waves_per_block(tuned["waves_per_block"])
-        ctx.set_team(2 if tuned.get("team") else 1)
+        if getattr(args, "save_settings", False):
+            save_settings(tuned, note="autotuned on %d of %d files" % (tuned["sample_files"], count))
+        cfg = tuned
+        source = "autotuned (%d-file sample)" % tuned["sample_files"]
+    else:
+        cfg = load_settings()
+        source = "cucm_amd/batch_settings.json"
+    s_max, tb = cfg["chunk_max"], cfg["table_bits"]
+    enc, dec = cfg["encoder"], cfg["decoder"]
+    spec = dec["spec_bits"]
     params = [make_params(pick_chunk_size(int(n), s_max), 128, 100, 0, tb) for n in sizes]
     d_in = ctx.alloc(total)
     d_in.upload(arena)
@@ -64,77 +97,206 @@ def batch_workload(args):
     cs = [d_c.addr + int(o) for o in coffs[:-1]]
     outs = [d_o.addr + int(o) for o in offs]
     lens = [int(n) for n in sizes]
-    # batched: one persistent launch each way
-    for _ in range(max(args.warmup, 0)):
+
+    def step():
+        apply_launch(ctx, enc)
+        a = time.perf_counter()
         cl, st = ctx.compress_many_device(ins, lens, params, cs, caps)
-        ctx.decompress_many_device(cs, cl, outs, lens, spec)
+        b = time.perf_counter()
+        sc = ctx.stats()
+        apply_launch(ctx, dec)
+        dl, dst = ctx.decompress_many_device(cs, cl, outs, lens, spec)
+        c = time.perf_counter()
+        sd = ctx.stats()
+        bad = [(i, x, y) for i, (x, y) in enumerate(zip(st, dst)) if x or y]
+        if bad:
+            raise RuntimeError("batch: %d files failed, first (file, compress, decompress) "
+                               "status: %s" % (len(bad), bad[:5]))
+        return cl, b - a, c - b, sc, sd
+
+    for i in range(args.warmup):
+        step()
+        log("warmup %d done" % i)
     ctx.synchronize()
-    a = time.perf_counter()
-    cl, st = ctx.compress_many_device(ins, lens, params, cs, caps)
-    b = time.perf_counter()
-    sc = ctx.stats()
-    dl, dst = ctx.decompress_many_device(cs, cl, outs, lens, spec)
-    c = time.perf_counter()
-    sd = ctx.stats()
-    bad = [(i, a, b) for i, (a, b) in enumerate(zip(st, dst)) if a or b]
-    if bad:
-        raise RuntimeError("batch: %d files failed, first (file, compress, decompress) status: %s"
-                           % (len(bad), bad[:5]))
+    tc, td, kc, kd = [], [], [], []
+    t_all0 = time.perf_counter()
+    for i in range(args.steps):
+        cl, x, y, sc, sd = step()
+        tc.append(x)
+        td.append(y)
+        kc.append(sc["codec_ms"])
+        kd.append(sd["codec_ms"])
+        log("step %d: compress %.3fs (kernel %.1f ms)  decompress %.3fs (kernel %.1f ms)"
+            % (i, x, sc["codec_ms"], y, sd["codec_ms"]))
+    ctx.synchronize()
+    t_all = time.perf_counter() - t_all0
     back = np.frombuffer(d_o.download(total), dtype=np.uint8)
     lossless = bool(np.array_equal(back, arena))
     comp_total = int(sum(cl))
-    # plain: one launch per file on a seeded sample of the same files
+    res = {
+        "metric": METRIC, "value": total / 1e6 / (t_all / args.steps), "unit": "MB/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * t_all / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32 fixed-point",
+        "data": "synthetic (seeded generator, resident in HBM)",
+        "config": {"workload": "batch: %d independent files, lognormal sizes (median 64 KiB, "
+                               "sigma 1.5, clipped to [512 B, 64 MiB]), order-2 Markov content "
+                               "(file i seeded 1000 + i); chunk = min(file, %d KiB), 2^%d "
+                               "slots/order, U=128, P=0; one persistent launch per direction"
+                               % (count, s_max // 1024, tb),
+                   "files": count, "bytes": total, "chunks": int(sc["tasks"]),
+                   "chunk_max": s_max, "table_bits": tb, "unit_bits": 128, "batch_chunks": 100,
+                   "encoder_launch": enc, "decoder_launch": dec, "settings": source},
+        "compress_MBps": total / 1e6 / float(np.median(tc)),
+        "decompress_MBps": total / 1e6 / float(np.median(td)),
+        "compress_kernel_ms": float(np.median(kc)),
+        "decompress_kernel_ms": float(np.median(kd)),
+        "ratio": total / comp_total, "lossless": lossless, "container_bytes": comp_total,
+        "compress_waves": int(sc["waves"]), "compress_teams": int(sc["teams"]),
+        "decompress_waves": int(sd["waves"]), "decompress_teams": int(sd["teams"]),
+        "model_bytes": int(max(sc["model_bytes"], sd["model_bytes"])),
+    }
+    res["compress_kernel_MBps"] = total / 1e3 / res["compress_kernel_ms"]
+    res["decompress_kernel_MBps"] = total / 1e3 / res["decompress_kernel_ms"]
+    # dominant kernel: the slower launch
+    if res["decompress_kernel_ms"] >= res["compress_kernel_ms"]:
+        kname, kms = batch_kernel_name(dec, "decoder"), res["decompress_kernel_ms"]
+    else:
+        kname, kms = batch_kernel_name(enc, "encoder"), res["compress_kernel_ms"]
+    achieved = ALG_BYTES_PER_INPUT_BYTE * total / (kms / 1e3) / 1e9
+    res["roofline"] = dict({"bound": "hbm", "kernel": kname, "achieved": achieved,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS},
+                           **batch_traffic(kname, total, kms / 1e3))
+    # plain path: one launch per file on a seeded sample of the same files
+    plain_files = getattr(args, "plain_files", 40)
+    if plain_files:
+        res.update(plain_sample(ctx, ins, lens, params, cs, caps, outs, cl, enc, dec, spec,
+                                plain_files, count))
+        res["batched_vs_plain_compress"] = res["compress_MBps"] / res["plain_compress_MBps"]
+        res["batched_vs_plain_decompress"] = res["decompress_MBps"] / res["plain_decompress_MBps"]
+    res["cpu_baseline"] = None
+    res["oracle_sample_identical"] = None
+    if not getattr(args, "no_cpu", False):
+        res["cpu_baseline"], res["oracle_sample_identical"] = batch_cpu_baseline(
+            arena, offs, sizes, d_c, coffs, cl, s_max, tb, args)
+    return res
+
+
+def batch_traffic(kernel, n, seconds):
+    """HBM bytes per launch of the dominant kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE / TCC_HIT / TCC_MISS runs of
+    the same command (FETCH_SIZE scaled by the copy-kernel calibration), with
+    the physical rate and the L2 hit rate."""
+    if not os.path.exists(BATCH_TRAFFIC):
+        return {"traffic": None}
+    d = json.load(open(BATCH_TRAFFIC))
+    t = d.get("kernels", {}).get(kernel)
+    if t is None or t.get("input_bytes") != n:
+        return {"traffic": None}
+    traffic = t["fetch_bytes"] + t["write_bytes"]
+    out = {"traffic": traffic, "hbm_GBps": traffic / seconds / 1e9,
+           "hbm_frac_of_peak": traffic / seconds / 1e9 / HBM_PEAK_GBS,
+           "l2_hit_rate": t.get("l2_hit_rate"), "traffic_source": d.get("source")}
+    ceil = d.get("ceiling")
+    if ceil and t.get("tcc_miss"):
+        lines_per_s = t["tcc_miss"] / seconds
+        out["line_fetches_per_s"] = lines_per_s
+        out["ceiling"] = ceil.get("name")
+        out["ceiling_frac"] = lines_per_s / ceil["lines_per_s"]
+    return out
+
+
+def plain_sample(ctx, ins, lens, params, cs, caps, outs, cl, enc, dec, spec, k, count):
+    from .autotune import apply_launch
     rng = np.random.default_rng(7)
-    sample = rng.choice(count, size=min(args.plain_files, count), replace=False)
+    sample = rng.choice(count, size=min(k, count), replace=False)
     sin = [ins[i] for i in sample]
     slen = [lens[i] for i in sample]
     sp = [params[i] for i in sample]
     scs = [cs[i] for i in sample]
     scap = [caps[i] for i in sample]
     souts = [outs[i] for i in sample]
+    apply_launch(ctx, enc)
     d = time.perf_counter()
     pl, _ = ctx.compress_many_device(sin, slen, sp, scs, scap, plain=True)
     e = time.perf_counter()
-    pdl, _ = ctx.decompress_many_device(scs, pl, souts, slen, spec, plain=True)
+    apply_launch(ctx, dec)
+    ctx.decompress_many_device(scs, pl, souts, slen, spec, plain=True)
     f = time.perf_counter()
-    plain_identical = all(pl[k] == cl[i] for k, i in enumerate(sample))
     sample_bytes = sum(slen)
-    res = {
-        "metric": "compress/decompress MB/s on 1 GPU at identical ratio; batched vs plain; p99",
-        "value": total / 1e6 / ((b - a) + (c - b)),
-        "unit": "MB/s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
-        "ms_per_step": 1000 * ((b - a) + (c - b)), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u8/int32 fixed-point",
-        "data": "synthetic (seeded generator, resident in HBM)",
-        "config": {"workload": "batch: %d lognormal files (median 64 KiB, sigma 1.5), 2^%d slots, "
-                               "chunk = min(file, %d KiB), U=128%s" % (
-                                   count, tb, s_max // 1024, ", autotuned" if tuned else ""),
-                   "files": count, "bytes": total, "table_bits": tb, "chunk_max": s_max,
-                   "spec_bits": spec,
-                   "waves_per_block": tuned["waves_per_block"] if tuned else 0,
-                   "team": bool(tuned.get("team")) if tuned else False},
-        "batched_compress_MBps": total / 1e6 / (b - a),
-        "batched_decompress_MBps": total / 1e6 / (c - b),
-        "batched_compress_kernel_ms": sc["codec_ms"],
-        "batched_decompress_kernel_ms": sd["codec_ms"],
-        "plain_sample_files": len(sample), "plain_sample_bytes": sample_bytes,
-        "plain_compress_MBps": sample_bytes / 1e6 / (e - d),
-        "plain_decompress_MBps": sample_bytes / 1e6 / (f - e),
-        "plain_identical_to_batched": bool(plain_identical),
-        "ratio": total / comp_total, "lossless": lossless, "tasks": sc["tasks"],
-        "waves": sc["waves"],
-    }
-    if tuned:
-        res["autotune"] = {k: tuned[k] for k in ("chunk_max", "table_bits", "spec_bits",
-                                                 "waves_per_block", "team", "sample_files",
-                                                 "sample_bytes", "ratio_tol", "seconds")}
-        res["autotune"]["trials"] = [{k: t.get(k) for k in ("chunk_max", "table_bits",
-                                                            "spec_bits", "waves_per_block",
-                                                            "team", "ratio", "score")}
-                                     for t in tuned["stream_trials"] + tuned["launch_trials"]]
-    res["batched_vs_plain_compress"] = res["batched_compress_MBps"] / res["plain_compress_MBps"]
-    res["batched_vs_plain_decompress"] = res["batched_decompress_MBps"] / res["plain_decompress_MBps"]
-    return res
+    return {"plain_sample_files": len(sample), "plain_sample_bytes": int(sample_bytes),
+            "plain_compress_MBps": sample_bytes / 1e6 / (e - d),
+            "plain_decompress_MBps": sample_bytes / 1e6 / (f - e),
+            "plain_identical_to_batched": bool(all(pl[j] == cl[i] for j, i in enumerate(sample)))}
+
+
+def _median_time(fn, runs=3):
+    fn()  # warm-up
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def batch_cpu_baseline(arena, offs, sizes, d_c, coffs, cl, s_max, tb, args):
+    """The oracle (the CPU restatement of the reference model) on size-stratified
+    samples of the same files, one file per task (OpenMP over files, each file
+    single-threaded): on every host thread the run may use, and on one; one
+    warm-up, then the median of 3 runs per direction.  Its containers are
+    compared with the GPU's for the same files."""
+    from oracle import oracle as O
+    from .autotune import stratified_sample
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+        (os.cpu_count() or 1)
+    out = {"unit": "MB/s (compress)", "kind": "port", "cores": threads}
+    same = True
+    for label, nthr, budget in (("all", threads, args.cpu_mib * MiB),
+                                ("one", 1, max(args.cpu_mib * MiB // 12, 1 << 20))):
+        # files small enough that one file per task balances over nthr
+        # threads, taken across their size distribution, largest first
+        cap_file = max(budget // (2 * nthr), 64 * 1024)
+        pool = np.nonzero(sizes <= cap_file)[0]
+        sub_idx = stratified_sample(sizes[pool], budget, seed=11)
+        idx = sorted((int(pool[j]) for j in sub_idx), key=lambda i: -int(sizes[i]))
+        sl = np.array([int(sizes[i]) for i in idx], dtype=np.uint64)
+        so = np.concatenate([[0], np.cumsum(sl)])[:-1].astype(np.uint64)
+        sub = np.concatenate([arena[int(offs[i]):int(offs[i]) + int(sizes[i])] for i in idx])
+        res = {}
+
+        def comp():
+            res["c"] = O.compress_files(sub, so, sl, s_max, 128, 100, tb, threads=nthr)
+
+        tcomp = _median_time(comp)
+        blobs, bo, bl = res["c"]
+
+        def decomp():
+            res["d"] = O.decompress_files(blobs, bo, bl, sl, threads=nthr)
+
+        tdec = _median_time(decomp)
+        ok = bool(np.array_equal(res["d"], sub))
+        for j, i in enumerate(idx):
+            gpu = d_c.download(int(cl[i]), offset=int(coffs[i]))
+            if gpu != blobs[int(bo[j]):int(bo[j] + bl[j])].tobytes():
+                same = False
+        mb = float(sl.sum()) / 1e6
+        if label == "all":
+            out.update({"value": mb / tcomp, "decompress_MBps": mb / tdec, "lossless": ok,
+                        "sample": "%d of the files up to %d KiB (%.1f MB, size-stratified), "
+                                  "oracle compress_files/decompress_files, one file per task "
+                                  "on %d threads, largest first; warm-up + median of 3"
+                                  % (len(idx), cap_file // 1024, mb, nthr)})
+        else:
+            out.update({"one_thread_compress_MBps": mb / tcomp,
+                        "one_thread_decompress_MBps": mb / tdec,
+                        "one_thread_sample": "%d files up to %d KiB (%.1f MB), 1 thread; "
+                                             "warm-up + median of 3"
+                                             % (len(idx), cap_file // 1024, mb)})
+            out["lossless"] = out["lossless"] and ok
+        log("cpu baseline (%s, %d threads): %.2f / %.2f MB/s" % (label, nthr, mb / tcomp,
+                                                                mb / tdec))
+    return out, same
 
 
 def _pct(xs, q):

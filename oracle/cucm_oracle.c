@@ -1031,3 +1031,65 @@ int orc_decompress(const uint8_t *in, uint64_t len, int threads, uint8_t *out,
   *out_len = N;
   return ORC_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* Many independent files, one file per task                           */
+/* ------------------------------------------------------------------ */
+
+/* Compresses nfiles files of the arena (file i at offs[i], lens[i] bytes)
+ * into one container each, in parallel over files (OpenMP dynamic, one file
+ * per task, each file coded single-threaded), with chunk size
+ * min(lens[i], chunk_max) as the batched GPU path cuts them.  Containers go
+ * to out + ooffs[i] (capacity ocaps[i]); their sizes to out_lens[i].  This
+ * is compress_file (cucm_compress_v2.cu:1151-1250) run once per file. */
+int orc_compress_files(const uint8_t *arena, const uint64_t *offs, const uint64_t *lens,
+                       uint32_t nfiles, uint32_t chunk_max, uint32_t U, uint32_t batch_num,
+                       int tb, int threads, uint8_t *out, const uint64_t *ooffs,
+                       const uint64_t *ocaps, uint64_t *out_lens) {
+  if (!arena || !offs || !lens || !out || !ooffs || !ocaps || !out_lens || chunk_max == 0)
+    return ORC_E_PARAM;
+  int err = ORC_OK;
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < (int64_t)nfiles; ++i) {
+    const uint64_t n = lens[i];
+    const uint32_t S = (uint32_t)(n < chunk_max ? (n ? n : 1) : chunk_max);
+    const int rc = orc_compress(arena + offs[i], n, S, U, batch_num, 0, tb, 1, 1,
+                                out + ooffs[i], ocaps[i], &out_lens[i]);
+    if (rc) {
+#ifdef _OPENMP
+#pragma omp critical(orc_err)
+#endif
+      err = rc;
+    }
+  }
+  (void)threads;
+  return err;
+}
+
+/* Decompresses nfiles containers (container i at in + ioffs[i], ilens[i]
+ * bytes) into out + ooffs[i] (capacity ocaps[i]), in parallel over files. */
+int orc_decompress_files(const uint8_t *in, const uint64_t *ioffs, const uint64_t *ilens,
+                         uint32_t nfiles, int threads, uint8_t *out, const uint64_t *ooffs,
+                         const uint64_t *ocaps, uint64_t *out_lens) {
+  if (!in || !ioffs || !ilens || !out || !ooffs || !ocaps || !out_lens) return ORC_E_PARAM;
+  int err = ORC_OK;
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < (int64_t)nfiles; ++i) {
+    const int rc = orc_decompress(in + ioffs[i], ilens[i], 1, out + ooffs[i], ocaps[i],
+                                  &out_lens[i]);
+    if (rc) {
+#ifdef _OPENMP
+#pragma omp critical(orc_err)
+#endif
+      err = rc;
+    }
+  }
+  (void)threads;
+  return err;
+}

@@ -156,6 +156,15 @@ int orc_compress_frozen(const uint8_t *in, uint64_t N, uint32_t S, uint32_t U,
 int orc_decompress(const uint8_t *in, uint64_t len, int threads, uint8_t *out,
                    uint64_t cap, uint64_t *out_len);
 
+/* ---- many files, one file per task (OpenMP over files) --------------- */
+int orc_compress_files(const uint8_t *arena, const uint64_t *offs, const uint64_t *lens,
+                       uint32_t nfiles, uint32_t chunk_max, uint32_t U, uint32_t batch_num,
+                       int tb, int threads, uint8_t *out, const uint64_t *ooffs,
+                       const uint64_t *ocaps, uint64_t *out_lens);
+int orc_decompress_files(const uint8_t *in, const uint64_t *ioffs, const uint64_t *ilens,
+                         uint32_t nfiles, int threads, uint8_t *out, const uint64_t *ooffs,
+                         const uint64_t *ocaps, uint64_t *out_lens);
+
 #ifdef __cplusplus
 }
 #endif

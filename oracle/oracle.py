@@ -73,6 +73,8 @@ def lib():
         L.orc_encode_batch_frozen.argtypes = [u8p, u64, u64, vp, ci, vp, u64, vp]
         L.orc_decode_batch_frozen.argtypes = [u8p, u64, u64, u64, vp, ci, vp]
         L.orc_compress_frozen.argtypes = [u8p, u64, u32, u32, u32, u32, ci, ci, vp, u64, vp]
+        L.orc_compress_files.argtypes = [vp, vp, vp, u32, u32, u32, u32, ci, ci, vp, vp, vp, vp]
+        L.orc_decompress_files.argtypes = [vp, vp, vp, u32, ci, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -282,3 +284,44 @@ def prelearn_decode(payload: bytes, P: int, model: Model) -> bytes:
     out = ctypes.create_string_buffer(max(P, 1))
     _check(lib().orc_prelearn_decode(payload, len(payload), P, model.ptr, out), "prelearn_decode")
     return out.raw[:P]
+
+
+def compress_files(arena, offs, lens, chunk_max, U=128, batch_num=100, tb=18, threads=0):
+    """One container per file (chunk size min(len, chunk_max)), files coded in
+    parallel, one file per task; returns (containers, offsets, lengths) with
+    the containers packed in one numpy arena."""
+    import numpy as np
+    arena = np.ascontiguousarray(arena, dtype=np.uint8)
+    n = len(lens)
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.uint64)
+    caps = np.array([compress_bound_one(int(l), chunk_max, batch_num) for l in lens], dtype=np.uint64)
+    ooffs = np.concatenate([[0], np.cumsum(caps)])[:-1].astype(np.uint64)
+    out = np.empty(int(caps.sum()) or 1, dtype=np.uint8)
+    olens = np.zeros(n, dtype=np.uint64)
+    _check(lib().orc_compress_files(arena.ctypes.data, offs.ctypes.data, lens.ctypes.data, n,
+                                    chunk_max, U, batch_num, tb, threads, out.ctypes.data,
+                                    ooffs.ctypes.data, caps.ctypes.data, olens.ctypes.data),
+           "compress_files")
+    return out, ooffs, olens
+
+
+def decompress_files(blobs, ioffs, ilens, sizes, threads=0):
+    """Inverse of compress_files: returns the files packed back to back."""
+    import numpy as np
+    n = len(ilens)
+    sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
+    ooffs = np.concatenate([[0], np.cumsum(sizes)])[:-1].astype(np.uint64)
+    out = np.empty(int(sizes.sum()) or 1, dtype=np.uint8)
+    olens = np.zeros(n, dtype=np.uint64)
+    ioffs = np.ascontiguousarray(ioffs, dtype=np.uint64)
+    ilens = np.ascontiguousarray(ilens, dtype=np.uint64)
+    _check(lib().orc_decompress_files(blobs.ctypes.data, ioffs.ctypes.data, ilens.ctypes.data, n,
+                                      threads, out.ctypes.data, ooffs.ctypes.data,
+                                      sizes.ctypes.data, olens.ctypes.data), "decompress_files")
+    return out[: int(sizes.sum())]
+
+
+def compress_bound_one(n, chunk_max, batch_num=100):
+    S = max(1, min(n, chunk_max))
+    return lib().orc_compress_bound(n, S, 0, batch_num)
