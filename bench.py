@@ -296,7 +296,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batch",
-                    choices=["stream", "structured", "small", "batch", "serve", "gather", "copy"])
+                    choices=["stream", "structured", "small", "batch", "serve", "gather", "copy",
+                             "replay"])
     ap.add_argument("--mib", type=int, default=0, help="input size (0 = the workload's own)")
     ap.add_argument("--spec", type=int, default=4)
     ap.add_argument("--frozen", action="store_true",
@@ -332,6 +333,9 @@ def main():
     elif args.workload == "gather":
         from cucm_amd.workloads import gather_workload
         line = gather_workload(args)
+    elif args.workload == "replay":
+        from cucm_amd.workloads import replay_workload
+        line = replay_workload(args)
     elif args.workload == "copy":
         from cucm_amd.workloads import copy_workload
         line = copy_workload(args)

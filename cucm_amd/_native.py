@@ -175,6 +175,8 @@ def probe_lib():
         P = ctypes.POINTER
         L.cucm_probe_gather.argtypes = [vp, u64, u32, u32, u32, u32, u32, P(ctypes.c_double), P(u64)]
         L.cucm_probe_copy.argtypes = [vp, vp, u64, P(ctypes.c_double)]
+        L.cucm_probe_replay_decode.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32,
+                                               P(ctypes.c_double)]
         _probe = L
     return _probe
 

@@ -20,6 +20,19 @@ int cucm_probe_gather(void *table, uint64_t bytes, uint32_t blocks, uint32_t thr
 /* Streaming device copy of `bytes` (multiple of 16), used to calibrate the
  * HBM traffic counters. */
 int cucm_probe_copy(void *dst, const void *src, uint64_t bytes, double *ms);
+/* The decoder's state-gather stream replayed without its model arithmetic:
+ * n chunks (device addresses h_src[i] of h_len[i] bytes), D-bit speculative
+ * rounds (2^D - 1 paths x 7 gathers, each round dependent on the last), the
+ * unit's slots written back every U bits, `waves` resident waves (a multiple
+ * of 4) each with its own model slot of 7 x 2^tb bytes.  flags bit 0: zero
+ * the slot per chunk; bit 1: skip the write-backs; bit 2: pack a round's
+ * (path, order) gathers across lanes, one instruction per 9 paths; bit 3:
+ * re-gather the unit's slots before its write-backs; bit 4: write each
+ * round's first-path slots right after their gathers instead of at the unit
+ * end.  *ms: kernel time. */
+int cucm_probe_replay_decode(const uint64_t *h_src, const uint32_t *h_len, uint32_t n,
+                             uint32_t tb, uint32_t D, uint32_t U, uint32_t waves,
+                             uint32_t flags, double *ms);
 #ifdef __cplusplus
 }
 #endif

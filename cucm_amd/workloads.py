@@ -198,11 +198,10 @@ def batch_traffic(kernel, n, seconds):
            "hbm_frac_of_peak": traffic / seconds / 1e9 / HBM_PEAK_GBS,
            "l2_hit_rate": t.get("l2_hit_rate"), "traffic_source": d.get("source")}
     ceil = d.get("ceiling")
-    if ceil and t.get("tcc_miss"):
-        lines_per_s = t["tcc_miss"] / seconds
-        out["line_fetches_per_s"] = lines_per_s
-        out["ceiling"] = ceil.get("name")
-        out["ceiling_frac"] = lines_per_s / ceil["lines_per_s"]
+    if ceil:
+        # the same access stream replayed without the model arithmetic
+        out["replay_ceiling_MBps"] = ceil["MBps"]
+        out["replay_ceiling_frac"] = n / 1e6 / seconds / ceil["MBps"]
     return out
 
 
@@ -454,6 +453,53 @@ def gather_workload(args):
                      "clocks_per_step_per_wave": clk.value / nw / steps, "ms": ms.value}
     return {"metric": "random one-byte touch rate", "footprint_bytes": bytes_,
             "model_footprint_bytes": footprint, "results": res}
+
+
+def replay_workload(args):
+    """The decoder's gather stream on the uneven-size batch (chunks of at most
+    128 KiB, 2^18 slots, U = 128) replayed by libcucm_probe.so without the
+    model arithmetic, at several numbers of resident chunks and speculation
+    widths: the memory-system ceiling the decoder's access pattern allows."""
+    import ctypes
+    from ._native import probe_lib
+    count = getattr(args, "files", 10000) or 10000
+    sizes, arena, offs = batch_files(count)
+    total = int(sizes.sum())
+    ctx = Context()
+    d_in = ctx.alloc(total)
+    d_in.upload(arena)
+    src, lens = [], []
+    for o, n in zip(offs, sizes):
+        o, n = int(o), int(n)
+        for c in range(0, n, 131072):
+            src.append(d_in.addr + o + c)
+            lens.append(min(131072, n - c))
+    srca = (ctypes.c_uint64 * len(src))(*src)
+    lena = (ctypes.c_uint32 * len(lens))(*lens)
+    out = []
+    # (speculation bits, resident waves, flags: 1 zero the slot per chunk, 2 no write-backs,
+    # 4 pack each round's gathers into one instruction, 8 re-gather before the write-backs,
+    # 16 write right after each round's gathers)
+    grid = getattr(args, "replay_grid", None) or [
+        (1, 1280, 1), (1, 2560, 1), (1, 5120, 1), (2, 1280, 1), (2, 2048, 1), (4, 1280, 1), (2, 1280, 3),
+        (1, 2560, 3), (2, 1280, 0), (1, 2560, 2), (1, 1280, 5), (1, 2560, 5), (1, 5120, 5), (2, 1280, 5),
+        (2, 2048, 5), (2, 4096, 5), (4, 1280, 5), (4, 2048, 5), (2, 2048, 7), (1, 2560, 7), (2, 1280, 9),
+        (2, 2048, 9), (1, 2560, 9), (2, 1280, 17), (1, 2560, 17), (2, 2048, 13),
+    ]
+    for D, waves, flags in grid:
+        ms = ctypes.c_double()
+        rc = probe_lib().cucm_probe_replay_decode(srca, lena, len(src), 18, D, 128, waves, flags,
+                                                  ctypes.byref(ms))
+        if rc:
+            raise RuntimeError("replay probe failed: %d" % rc)
+        out.append({"spec_bits": D, "waves": waves, "zero_slot": bool(flags & 1),
+                    "write_back": not (flags & 18), "packed_lanes": bool(flags & 4),
+                    "regather_before_write": bool(flags & 8), "write_per_round": bool(flags & 16),
+                    "ms": ms.value, "MBps": total / 1e3 / ms.value})
+        log("replay D=%d waves=%d flags=%d: %.0f ms, %.1f MB/s" % (D, waves, flags, ms.value,
+                                                                  out[-1]["MBps"]))
+    return {"metric": "decoder gather-stream replay (no model arithmetic)", "bytes": total,
+            "chunks": len(src), "table_bits": 18, "unit_bits": 128, "results": out}
 
 
 def copy_workload(args):
