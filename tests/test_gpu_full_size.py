@@ -7,7 +7,10 @@ launch each decoded back to itself.
   * 256 MiB mixed stream, 1 MiB chunks, 2^22 slots per order (the metric's
     single-GPU configuration);
   * 1 GiB structured binary, 256 KiB chunks (4096 chunks), 2^22;
-  * 10,000 lognormal files (median 64 KiB / 32, sigma 1.5) in one launch.
+  * 10,000 lognormal files (median 64 KiB / 32, sigma 1.5) in one launch;
+  * the batched configuration at its full size: 10,000 lognormal files
+    (median 64 KiB, sigma 1.5, clipped to [512 B, 64 MiB]; 1.96 GiB) with the
+    bench's settings, device-resident, one launch each way.
 """
 import struct
 
@@ -88,3 +91,48 @@ def test_ten_thousand_files_one_launch(ctx, oracle):
     assert ctx.stats()["launches"] == 1
     bad = [i for i, (a, b) in enumerate(zip(files, back)) if a != b]
     assert not bad, bad[:10]
+
+
+def test_batch_configuration_full_size(ctx, oracle):
+    """All 10,000 files compressed in one launch and decompressed in another
+    (the bench's settings and launch shapes), every file lossless, and a
+    size-stratified sample of 24 files plus the largest byte-identical to the
+    oracle's containers."""
+    from cucm_amd.autotune import apply_launch, load_settings, stratified_sample
+    from cucm_amd.codec import output_cap
+    from cucm_amd.workloads import batch_files, pick_chunk_size
+    cfg = load_settings()
+    sizes, arena, offs = batch_files(10000)
+    total = int(sizes.sum())
+    assert sizes.min() >= 512 and sizes.max() <= 64 * MiB and total > 1900 * MiB
+    S, tb = cfg["chunk_max"], cfg["table_bits"]
+    params = [make_params(pick_chunk_size(int(n), S), 128, 100, 0, tb) for n in sizes]
+    d_in = ctx.alloc(total)
+    d_in.upload(arena)
+    caps = [output_cap(int(n), p) for n, p in zip(sizes, params)]
+    coffs = np.concatenate([[0], np.cumsum(np.array(caps, dtype=np.uint64) + 255 & ~np.uint64(255))])
+    d_c = ctx.alloc(int(coffs[-1]))
+    d_o = ctx.alloc(total)
+    ins = [d_in.addr + int(o) for o in offs]
+    cs = [d_c.addr + int(o) for o in coffs[:-1]]
+    outs = [d_o.addr + int(o) for o in offs]
+    lens = [int(n) for n in sizes]
+    try:
+        apply_launch(ctx, cfg["encoder"])
+        cl, st = ctx.compress_many_device(ins, lens, params, cs, caps)
+        assert not any(st) and ctx.stats()["launches"] == 1
+        apply_launch(ctx, cfg["decoder"])
+        dl, dst = ctx.decompress_many_device(cs, cl, outs, lens, cfg["decoder"]["spec_bits"])
+        assert not any(dst) and ctx.stats()["launches"] == 1
+        assert list(dl) == lens
+        back = np.frombuffer(d_o.download(total), dtype=np.uint8)
+        assert bool(np.array_equal(back, arena))
+        pick = set(stratified_sample(sizes, 24 << 20, seed=5)[:24]) | {int(np.argmax(sizes))}
+        for i in sorted(pick):
+            n = int(sizes[i])
+            ref = oracle.compress(arena[int(offs[i]):int(offs[i]) + n].tobytes(),
+                                  pick_chunk_size(n, S), 128, 100, 0, tb)
+            assert d_c.download(int(cl[i]), offset=int(coffs[i])) == ref, "file %d (%d B)" % (i, n)
+    finally:
+        ctx.set_team(0)
+        ctx.set_waves_per_block(0)

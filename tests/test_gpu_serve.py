@@ -84,3 +84,52 @@ def test_serve_poll_nowait_and_errors(ctx):
         sc.serve_stop()
     finally:
         sc.close()
+
+
+def test_serve_poisson_mix_of_the_serving_configuration(ctx):
+    """The serving configuration's request mix: Poisson arrivals, half
+    compress requests of fresh payloads and half decompress requests of
+    containers made earlier, sizes min(round(256 KiB * exp(1.2 Z)), 128 MiB)
+    with order-2 Markov content, 16 KiB chunks and 2^18 slots.  Requests are
+    submitted from a thread at their arrival times while earlier ones run;
+    every result equals the same request run alone."""
+    import threading
+    import time
+    count = 48
+    sizes = datagen.lognormal_sizes(count, 262144, 1.2, 1, 128 << 20, 13)
+    pool_sizes = datagen.lognormal_sizes(count // 2, 262144, 1.2, 1, 128 << 20, 17)
+    arena, offs = datagen.markov2_files(sizes, seed0=5000)
+    parena, poffs = datagen.markov2_files(pool_sizes, seed0=9000)
+    fresh = [arena[int(o):int(o) + int(n)].tobytes() for o, n in zip(offs, sizes)]
+    pool = [parena[int(o):int(o) + int(n)].tobytes() for o, n in zip(poffs, pool_sizes)]
+    p = make_params(16384, 128, 100, 0, 18)
+    blobs = ctx.compress_many(pool, p)
+    rng = np.random.default_rng(11)
+    ops = rng.random(count) < 0.5
+    reqs = [("c", fresh[i], ctx.compress(fresh[i], p)) if ops[i] else
+            ("d", blobs[i % len(blobs)], pool[i % len(pool)]) for i in range(count)]
+    mean = float(np.mean([len(r[1]) if r[0] == "c" else len(r[2]) for r in reqs]))
+    arrivals = datagen.poisson_arrivals(count, 40e6 / mean, 19)  # ~40 MB/s offered
+    sc = cucm_amd.Context()
+    ids = [None] * count
+    try:
+        sc.serve_start()
+
+        def submitter():
+            t0 = time.perf_counter()
+            for i, (kind, x, _) in enumerate(reqs):
+                d = t0 + arrivals[i] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[i] = sc.submit_compress(x, p) if kind == "c" else sc.submit_decompress(x)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        for i, (kind, x, want) in enumerate(reqs):
+            out, info = sc.poll(ids[i], wait=True)
+            assert out == want, "request %d (%s, %d B) differs from the request alone" % (
+                i, kind, len(x))
+        sc.serve_stop()
+    finally:
+        sc.close()
