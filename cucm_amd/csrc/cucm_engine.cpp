@@ -194,7 +194,7 @@ int validate_params(const cucm_params &p, uint64_t n) {
   if (p.table_bits < 8 || p.table_bits > 25) return CUCM_E_PARAM;
   if (p.flags & ~(CUCM_FLAG_NO_TAIL_PAD | CUCM_FLAG_FROZEN)) return CUCM_E_PARAM;
   if (p.prelearn_size > n) return CUCM_E_PARAM;
-  if (p.unit_bits > kMaxUnitBits) return CUCM_E_UNSUPPORTED;
+  if (p.unit_bits > kMaxBigUnitBits) return CUCM_E_PARAM;
   return CUCM_OK;
 }
 
@@ -275,14 +275,71 @@ Shape frozen_shape(const cucm_ctx *c, uint64_t ntasks) {
   return s;
 }
 
+// Large-unit tasks: one wave per chunk, up to 4 waves per workgroup sharing
+// the tables, each wave with a model slot and a region of unit records
+// (per_wave bytes together), within mem_left.
+Shape big_shape(const cucm_ctx *c, uint64_t ntasks, uint64_t per_wave, uint64_t mem_left) {
+  Shape s;
+  uint32_t wmax = 4;
+  while (wmax > 1 && big_lds_bytes(wmax) > c->lds_block) --wmax;
+  const uint64_t per_cu = std::max<uint64_t>(1, c->lds_cu / big_lds_bytes(wmax));
+  uint64_t waves = std::min<uint64_t>(ntasks, (uint64_t)c->cus * per_cu * wmax);
+  if (c->max_waves) waves = std::min<uint64_t>(waves, c->max_waves);
+  if (per_wave) waves = std::min<uint64_t>(waves, mem_left / per_wave);
+  if (waves == 0) return s;
+  s.wpb = (uint32_t)std::min<uint64_t>(wmax, waves);
+  s.blocks = (uint32_t)(waves / s.wpb);
+  return s;
+}
+
+int plan_pass(cucm_ctx *c, const PassCounts &n, uint32_t spec, bool dec, uint64_t mem_left,
+              PassPlan &pl, cucm_stats &st) {
+  pl = PassPlan{};
+  pl.Umax = n.Umax;
+  pl.slot_bytes = align_up(7ull << n.tbmax, 256);
+  pl.big_bytes = n.nbig ? big_record_bytes(n.Ubig) : 0;
+  pl.Ubig = n.Ubig;
+  uint64_t slots = 0;
+  if (n.nmain) {
+    pl.sh = pick_shape(c, n.nmain, n.Umax, spec, dec, pl.slot_bytes, mem_left);
+    if (pl.sh.slots() == 0) return CUCM_E_BUDGET;
+    slots = pl.sh.slots();
+    if (pl.sh.team) st.teams = std::max<uint64_t>(st.teams, pl.sh.blocks);
+  }
+  if (n.nfrozen) pl.fz = frozen_shape(c, n.nfrozen);
+  if (n.nbig) {
+    // the large-unit launch runs after the others on the same stream and
+    // reuses their model slots
+    pl.bg = big_shape(c, n.nbig, pl.slot_bytes + pl.big_bytes, mem_left);
+    if (pl.bg.waves() == 0) return CUCM_E_BUDGET;
+    slots = std::max<uint64_t>(slots, pl.bg.waves());
+    CU_TRY(c->bigrec.reserve(pl.big_bytes * pl.bg.waves()));
+  }
+  if (slots) CU_TRY(c->pool.reserve(pl.slot_bytes * slots));
+  st.model_bytes = std::max<uint64_t>(st.model_bytes, pl.slot_bytes * slots);
+  st.waves = std::max<uint64_t>(st.waves, pl.sh.waves() + pl.fz.waves() + pl.bg.waves());
+  return CUCM_OK;
+}
+
+void plan_args(const cucm_ctx *c, const PassPlan &pl, CodecArgs &a) {
+  a.pool = (uint8_t *)c->pool.p;
+  a.slot_bytes = pl.slot_bytes;
+  a.big = (uint8_t *)c->bigrec.p;
+  a.big_bytes = pl.big_bytes;
+  a.big_umax = pl.Ubig;
+}
+
 // One codec pass over the tasks on the device: the unit codec (shape sh)
-// for the tasks without kTaskFrozen and the frozen codec (shape fz) for the
-// rest, back to back on the context's stream, each kernel pulling from its
-// own queue head (counter words 0 and 1) and skipping the other's tasks.
-// With `timed`, events 0 and 1 bracket both launches.
-int launch_pass(cucm_ctx *c, const CodecArgs &a, bool decode, uint32_t Umax, const Shape &sh,
-                const Shape &fz, bool timed, cucm_stats &st) {
-  HIP_TRY(hipMemsetAsync(a.counter, 0, 8, c->stream));
+// for the tasks without kTaskFrozen or kTaskBigUnit, the frozen codec (fz)
+// and the large-unit codec (bg) for those, back to back on the context's
+// stream, each kernel pulling from its own queue head (counter words 0, 1
+// and 2) and skipping the others' tasks.  With `timed`, events 0 and 1
+// bracket the launches.
+int launch_pass(cucm_ctx *c, const CodecArgs &a, bool decode, const PassPlan &pl, bool timed,
+                cucm_stats &st) {
+  const Shape &sh = pl.sh, &fz = pl.fz, &bg = pl.bg;
+  const uint32_t Umax = pl.Umax;
+  HIP_TRY(hipMemsetAsync(a.counter, 0, 16, c->stream));
   if (timed) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   if (sh.blocks) {
     HIP_TRY(launch_codec(decode, a, Umax, sh.blocks, sh.wpb, sh.team, c->stream));
@@ -292,6 +349,13 @@ int launch_pass(cucm_ctx *c, const CodecArgs &a, bool decode, uint32_t Umax, con
     CodecArgs f = a;
     f.counter = a.counter + 1;
     HIP_TRY(launch_frozen(decode, f, fz.blocks, fz.wpb, c->stream));
+    st.launches += 1;
+  }
+  if (bg.blocks) {
+    CodecArgs b = a;
+    b.counter = a.counter + 2;
+    b.prof = nullptr;
+    HIP_TRY(launch_big(decode, b, bg.blocks, bg.wpb, c->stream));
     st.launches += 1;
   }
   if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
@@ -350,8 +414,7 @@ static int prof_collect(cucm_ctx *c, const CodecArgs &a) {
 int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
   InitModels im;
   std::vector<Task> tasks;
-  uint32_t Umax = 32;
-  uint32_t tbmax = 8;
+  PassCounts cnt;
   uint64_t staging_total = 0;
   for (auto &J : jobs) {
     J.status = validate_params(J.p, J.n);
@@ -395,6 +458,7 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
     J.first = (uint32_t)tasks.size();
     const bool frozen = (J.p.flags & CUCM_FLAG_FROZEN) != 0;
     const bool pad = !(J.p.flags & CUCM_FLAG_NO_TAIL_PAD) && !frozen;
+    const bool big = !frozen && J.p.unit_bits > kMaxUnitBits;
     for (uint32_t ci = 0; ci < J.C; ++ci) {
       Task t{};
       t.src = (uint64_t)(uintptr_t)(J.din + P + (uint64_t)ci * S);
@@ -403,7 +467,8 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
       t.U = J.p.unit_bits;
       t.tb = J.p.table_bits;
       t.flags = (uint8_t)(((pad && ci == J.C - 1) ? kTaskTailPad : 0) |
-                          (J.model >= 0 ? kTaskInitModel : 0) | (frozen ? kTaskFrozen : 0));
+                          (J.model >= 0 ? kTaskInitModel : 0) | (frozen ? kTaskFrozen : 0) |
+                          (big ? kTaskBigUnit : 0));
       t.model = J.model >= 0 ? (uint32_t)J.model : 0;
       t.dst = staging_total;  // offset until the staging buffer is placed
       const uint64_t cap = staging_cap(S);
@@ -411,8 +476,7 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
       staging_total += cap;
       tasks.push_back(t);
     }
-    Umax = std::max<uint32_t>(Umax, J.p.unit_bits);
-    tbmax = std::max<uint32_t>(tbmax, J.p.table_bits);
+    cnt.add(J.C, frozen, big, J.p.unit_bits, J.p.table_bits);
   }
   const uint32_t ntasks = (uint32_t)tasks.size();
   st.tasks += ntasks;
@@ -435,34 +499,22 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
                            hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->tasks.p, tasks.data(), sizeof(Task) * ntasks, hipMemcpyHostToDevice,
                          c->stream));
-  const uint64_t slot_bytes = align_up(7ull << tbmax, 256);
   const uint64_t used = c->staging.cap + c->tasks.cap + c->results.cap + c->arena_in.cap +
                         c->arena_out.cap;
   const uint64_t mem_left = c->budget > used ? c->budget - used : 0;
-  uint32_t nfrozen = 0;
-  for (const Task &t : tasks) nfrozen += (t.flags & kTaskFrozen) ? 1u : 0u;
-  Shape sh, fz;
-  if (ntasks > nfrozen) {
-    sh = pick_shape(c, ntasks - nfrozen, Umax, 0, false, slot_bytes, mem_left);
-    if (sh.slots() == 0) return CUCM_E_BUDGET;
-    CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
-    if (sh.team) st.teams = std::max<uint64_t>(st.teams, sh.blocks);
-    st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
-  }
-  if (nfrozen) fz = frozen_shape(c, nfrozen);
-  st.waves = std::max<uint64_t>(st.waves, sh.waves() + fz.waves());
+  PassPlan pl;
+  CU_TRY(plan_pass(c, cnt, 0, false, mem_left, pl, st));
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
   a.ntasks = ntasks;
   a.spec_bits = 0;
   a.counter = (uint32_t *)c->counter.p;
-  a.pool = (uint8_t *)c->pool.p;
-  a.slot_bytes = slot_bytes;
   a.result = (uint64_t *)c->results.p;
   a.tabs = (const DevTables *)c->tabs.p;
   a.init = (const DevInitModel *)c->initm.p;
+  plan_args(c, pl, a);
   CU_TRY(prof_arm(c, a));
-  CU_TRY(launch_pass(c, a, false, Umax, sh, fz, true, st));
+  CU_TRY(launch_pass(c, a, false, pl, true, st));
   CU_TRY(prof_collect(c, a));
   std::vector<uint64_t> res(ntasks);
   HIP_TRY(hipMemcpyAsync(res.data(), c->results.p, 8ull * ntasks, hipMemcpyDeviceToHost, c->stream));
@@ -499,15 +551,13 @@ int encode_run(cucm_ctx *c, std::vector<EncJob> &jobs, cucm_stats &st) {
     b.ntasks = (uint32_t)rt.size();
     b.result = (uint64_t *)c->results2.p;
     b.prof = nullptr;
-    uint32_t rfrozen = 0;
-    for (const Task &x : rt) rfrozen += (x.flags & kTaskFrozen) ? 1u : 0u;
-    Shape s2, f2;
-    if (rt.size() > rfrozen) {
-      s2 = pick_shape(c, rt.size() - rfrozen, Umax, 0, false, slot_bytes, mem_left);
-      if (s2.slots() == 0 || s2.slots() > sh.slots()) return CUCM_E_BUDGET;
-    }
-    if (rfrozen) f2 = frozen_shape(c, rfrozen);
-    CU_TRY(launch_pass(c, b, false, Umax, s2, f2, false, st));
+    PassCounts rc2;
+    for (const Task &x : rt)
+      rc2.add(1, (x.flags & kTaskFrozen) != 0, (x.flags & kTaskBigUnit) != 0, x.U, x.tb);
+    PassPlan p2;
+    CU_TRY(plan_pass(c, rc2, 0, false, mem_left, p2, st));
+    plan_args(c, p2, b);
+    CU_TRY(launch_pass(c, b, false, p2, false, st));
     std::vector<uint64_t> r2(rt.size());
     HIP_TRY(hipMemcpyAsync(r2.data(), c->results2.p, 8 * rt.size(), hipMemcpyDeviceToHost,
                            c->stream));
@@ -621,8 +671,8 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
   // Chunk counts come from untrusted headers: each chunk needs at least a
   // 4-byte end offset and one payload byte, so a count above (len - seg) / 5
   // is corrupt; totals are kept in 64 bits and refused above 2^32 - 1.
-  uint64_t ntasks = 0, nfrozen = 0;
-  uint32_t Umax = 32, tbmax = 8;
+  uint64_t ntasks = 0;
+  PassCounts cnt;
   for (uint32_t j = 0; j < n; ++j) {
     DecJob &J = jobs[j];
     J.status = CUCM_OK;
@@ -632,7 +682,7 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
         J.status = CUCM_E_SMALL;
         continue;
       }
-      if (J.body > 0 && (J.S == 0 || J.U == 0 || J.batch == 0 || J.U > kMaxUnitBits)) {
+      if (J.body > 0 && (J.S == 0 || J.U == 0 || J.batch == 0 || J.U > kMaxBigUnitBits)) {
         J.status = CUCM_E_PARAM;
         continue;
       }
@@ -661,12 +711,12 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
       p.U = J.U;
       p.tb = J.tb;
       p.batch = J.batch;
-      p.flags = (model >= 0 ? kTaskInitModel : 0) | (J.frozen ? kTaskFrozen : 0);
+      const bool big = !J.frozen && J.U > (uint32_t)kMaxUnitBits;
+      p.flags = (model >= 0 ? kTaskInitModel : 0) | (J.frozen ? kTaskFrozen : 0) |
+                (big ? kTaskBigUnit : 0);
       p.model = model >= 0 ? (uint32_t)model : 0;
       ntasks += p.ntasks;
-      if (J.frozen) nfrozen += p.ntasks;
-      Umax = std::max(Umax, J.U);
-      tbmax = std::max(tbmax, J.tb);
+      cnt.add(p.ntasks, J.frozen, big, J.U, J.tb);
       pj.push_back(p);
       pj_of.push_back(j);
       continue;
@@ -696,10 +746,6 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
     }
     if (N > J.cap) {
       J.status = CUCM_E_SMALL;
-      continue;
-    }
-    if (body > 0 && U > kMaxUnitBits) {
-      J.status = CUCM_E_UNSUPPORTED;
       continue;
     }
     const uint64_t C = body ? (body + S - 1) / S : 0;
@@ -744,12 +790,12 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
     p.U = U;
     p.tb = tb;
     p.batch = batch;
-    p.flags = (model >= 0 ? kTaskInitModel : 0) | (frozen ? kTaskFrozen : 0);
+    const bool big = !frozen && U > (uint32_t)kMaxUnitBits;
+    p.flags = (model >= 0 ? kTaskInitModel : 0) | (frozen ? kTaskFrozen : 0) |
+              (big ? kTaskBigUnit : 0);
     p.model = model >= 0 ? (uint32_t)model : 0;
     ntasks += p.ntasks;
-    if (frozen) nfrozen += p.ntasks;
-    Umax = std::max(Umax, U);
-    tbmax = std::max(tbmax, tb);
+    cnt.add(p.ntasks, frozen, big, U, tb);
     pj.push_back(p);
     pj_of.push_back(j);
   }
@@ -769,32 +815,22 @@ int decode_run(cucm_ctx *c, std::vector<DecJob> &jobs, uint32_t spec, cucm_stats
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(launch_build_decode_tasks((const ParseJob *)c->parsejobs.p, (uint32_t)pj.size(),
                                     (Task *)c->tasks.p, (int32_t *)c->status.p, c->stream));
-  const uint64_t slot_bytes = align_up(7ull << tbmax, 256);
   const uint64_t used = c->staging.cap + c->tasks.cap + c->results.cap + c->arena_in.cap +
                         c->arena_out.cap;
   const uint64_t mem_left = c->budget > used ? c->budget - used : 0;
-  Shape sh, fz;
-  if (ntasks > nfrozen) {
-    sh = pick_shape(c, ntasks - nfrozen, Umax, spec, true, slot_bytes, mem_left);
-    if (sh.slots() == 0) return CUCM_E_BUDGET;
-    CU_TRY(c->pool.reserve(slot_bytes * sh.slots()));
-    if (sh.team) st.teams = std::max<uint64_t>(st.teams, sh.blocks);
-    st.model_bytes = std::max<uint64_t>(st.model_bytes, slot_bytes * sh.slots());
-  }
-  if (nfrozen) fz = frozen_shape(c, nfrozen);
-  st.waves = std::max<uint64_t>(st.waves, sh.waves() + fz.waves());
+  PassPlan pl;
+  CU_TRY(plan_pass(c, cnt, spec, true, mem_left, pl, st));
   CodecArgs a{};
   a.tasks = (const Task *)c->tasks.p;
   a.ntasks = (uint32_t)ntasks;
   a.spec_bits = spec;
   a.counter = (uint32_t *)c->counter.p;
-  a.pool = (uint8_t *)c->pool.p;
-  a.slot_bytes = slot_bytes;
   a.result = (uint64_t *)c->results.p;
   a.tabs = (const DevTables *)c->tabs.p;
   a.init = (const DevInitModel *)c->initm.p;
+  plan_args(c, pl, a);
   CU_TRY(prof_arm(c, a));
-  CU_TRY(launch_pass(c, a, true, Umax, sh, fz, true, st));
+  CU_TRY(launch_pass(c, a, true, pl, true, st));
   CU_TRY(prof_collect(c, a));
   std::vector<int32_t> stv(pj.size());
   HIP_TRY(hipMemcpyAsync(stv.data(), c->status.p, 4 * pj.size(), hipMemcpyDeviceToHost, c->stream));
@@ -1375,7 +1411,6 @@ int cucm_decompress_batch(cucm_ctx *ctx, const uint8_t *blob, uint64_t blob_len,
   if (!ctx || !blob || !out || out_len == 0 || chunk_size == 0 || unit_bits == 0 ||
       unit_bits % 32 || unit_bits > 65535 || table_bits < 8 || table_bits > 25)
     return CUCM_E_PARAM;
-  if (unit_bits > kMaxUnitBits) return CUCM_E_UNSUPPORTED;
   const uint64_t C = (out_len + chunk_size - 1) / chunk_size;
   if (out_len > 0xffffffffull || C > 65535 || blob_len + 4 > 0xffffffffull) return CUCM_E_PARAM;
   // Wrap the blob in a one-batch container and decode it.

@@ -47,7 +47,7 @@ struct cucm_ctx {
   bool profile = false;
   uint64_t phase[cucm::kPhases] = {};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  DevBuf tabs, pool, staging, overflow, tasks, tasks2, results, results2, taskdst, asmjobs,
+  DevBuf tabs, pool, bigrec, staging, overflow, tasks, tasks2, results, results2, taskdst, asmjobs,
       headers, joblen, counter, initm, ptrs, hdrs, parsejobs, status, arena_in, arena_out, prof;
   PinnedBuf pin;
   std::vector<cucm::Task> last_tasks;
@@ -114,12 +114,43 @@ Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spe
                  uint64_t slot_bytes, uint64_t mem_left);
 // Shape of the frozen-model codec's launch (no model slots).
 Shape frozen_shape(const cucm_ctx *c, uint64_t ntasks);
-// One codec pass: the unit codec (sh) and the frozen codec (fz) back to back
-// on the context's stream, each pulling from its own queue head; the two
-// head words at a.counter must be zero.  With `timed`, events 0 and 1
-// bracket both launches.
-int launch_pass(cucm_ctx *c, const cucm::CodecArgs &a, bool decode, uint32_t Umax,
-                const Shape &sh, const Shape &fz, bool timed, cucm_stats &st);
+// Tasks of one codec pass by kind: the unit codec's (U <= 256), the frozen
+// codec's and the large-unit codec's, with the largest U of the first and
+// last kinds and the largest table bits.
+struct PassCounts {
+  uint64_t nmain = 0, nfrozen = 0, nbig = 0;
+  uint32_t Umax = 32, Ubig = 0, tbmax = 8;
+  void add(uint64_t n, bool frozen, bool big, uint32_t U, uint32_t tb) {
+    if (frozen)
+      nfrozen += n;
+    else if (big) {
+      nbig += n;
+      Ubig = U > Ubig ? U : Ubig;
+    } else {
+      nmain += n;
+      Umax = U > Umax ? U : Umax;
+    }
+    tbmax = tb > tbmax ? tb : tbmax;
+  }
+};
+// Launch shapes of a pass and the buffers they need.
+struct PassPlan {
+  Shape sh, fz, bg;  // unit codec, frozen codec, large-unit codec
+  uint32_t Umax = 32, Ubig = 0;
+  uint64_t slot_bytes = 0, big_bytes = 0;
+};
+Shape big_shape(const cucm_ctx *c, uint64_t ntasks, uint64_t per_wave, uint64_t mem_left);
+// Shapes within mem_left bytes, model slots and large-unit records reserved.
+int plan_pass(cucm_ctx *c, const PassCounts &n, uint32_t spec, bool dec, uint64_t mem_left,
+              PassPlan &pl, cucm_stats &st);
+// The plan's buffers in a's pool / slot / large-unit fields.
+void plan_args(const cucm_ctx *c, const PassPlan &pl, cucm::CodecArgs &a);
+// One codec pass: the unit codec, the frozen codec and the large-unit codec
+// back to back on the context's stream, each pulling from its own queue head
+// (a.counter[0..2], cleared here).  With `timed`, events 0 and 1 bracket the
+// launches.
+int launch_pass(cucm_ctx *c, const cucm::CodecArgs &a, bool decode, const PassPlan &pl,
+                bool timed, cucm_stats &st);
 // Staging capacity of one chunk's payload before assembly (real data stays
 // below ~1.01 S; an overflow is flagged in the chunk's result word).
 inline uint64_t staging_cap(uint64_t S) { return (S + S / 2 + 256 + 15) / 16 * 16; }

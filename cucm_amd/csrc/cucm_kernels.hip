@@ -2,6 +2,7 @@
 // codec: the persistent codec kernel (its per-wave code is in cucm_wave.h),
 // container assembly, device-side container parsing, and the host launchers.
 #define CUCM_WITH_LAUNCHERS
+#include "cucm_bigunit.h"
 #include "cucm_frozen.h"
 #include "cucm_team.h"
 
@@ -43,7 +44,7 @@ __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a) {
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
-    if (tk.flags & kTaskFrozen) continue;
+    if (tk.flags & (kTaskFrozen | kTaskBigUnit)) continue;
     if constexpr (kDecode)
       decode_task<NJ, D, kProf>(a, tk, t, M, T, slot, Umax, lane);
     else
@@ -73,7 +74,7 @@ __global__ void __launch_bounds__(256) decode_team_kernel(CodecArgs a) {
     team_sync();
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
-    if (tk.flags & kTaskFrozen) continue;  // uniform: every wave read the same task
+    if (tk.flags & (kTaskFrozen | kTaskBigUnit)) continue;  // uniform: every wave read the same task
     decode_team<NJ, D, kProf>(a, tk, t, X, T, slot, Umax, wid, lane, rid);
   }
 }
@@ -97,7 +98,7 @@ __global__ void __launch_bounds__(256) encode_team_kernel(CodecArgs a) {
     team_sync();
     if (t >= a.ntasks) break;
     const Task tk = a.tasks[t];
-    if (tk.flags & kTaskFrozen) continue;  // uniform: every wave read the same task
+    if (tk.flags & (kTaskFrozen | kTaskBigUnit)) continue;  // uniform: every wave read the same task
     encode_team<NJ, kProf>(a, tk, t, X, T, slot, Umax, wid, lane);
   }
 }
@@ -125,6 +126,33 @@ __global__ void __launch_bounds__(512) frozen_kernel(CodecArgs a) {
       frozen_decode_task<D>(a, tk, t, F, T, states, lane);
     else
       frozen_encode_task(a, tk, t, F, T, states, lane);
+  }
+}
+
+// The large-unit codec: one wave per chunk of the tasks marked kTaskBigUnit,
+// each wave with a model slot and a region of unit records in global memory.
+template <bool kDecode>
+__global__ void __launch_bounds__(256) bigunit_kernel(CodecArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  LDS DevTables *T = (LDS DevTables *)lds;
+  stage_tables(a.tabs, T);
+  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const BigMem M =
+      carve_big((LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * big_wave_bytes());
+  const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wid;
+  GLB uint8_t *slot = (GLB uint8_t *)a.pool + gw * a.slot_bytes;
+  const BigRec R = big_records((GLB uint8_t *)a.big + gw * a.big_bytes, a.big_umax);
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.counter, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= a.ntasks) break;
+    const Task tk = a.tasks[t];
+    if (!(tk.flags & kTaskBigUnit) || tk.L == 0 || tk.U > a.big_umax) continue;
+    if constexpr (kDecode)
+      big_decode_task(a, tk, t, M, T, slot, R, lane);
+    else
+      big_encode_task(a, tk, t, M, T, slot, R, lane);
   }
 }
 
@@ -278,7 +306,9 @@ __global__ void build_decode_tasks_kernel(const ParseJob *jobs, uint32_t n, Task
       t.U = (uint16_t)J.U;
       t.tb = (uint8_t)J.tb;
       t.S = J.S;
-      tasks[J.first_task + c] = t;  // L = 0: the decoder skips it
+      // L = 0: the decoder skips it; the kind flags keep it with its codec
+      t.flags = (uint8_t)(J.flags & (kTaskFrozen | kTaskBigUnit));
+      tasks[J.first_task + c] = t;
     }
   }
   status[j] = st;
@@ -431,6 +461,24 @@ hipError_t launch_frozen(bool decode, const CodecArgs &a, uint32_t blocks, uint3
     case 8: return launch_frozen_one<true, 8>(a, blocks, wpb, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+uint32_t big_lds_bytes(uint32_t waves) {
+  return align16(sizeof(DevTables)) + waves * big_wave_bytes();
+}
+
+hipError_t launch_big(bool decode, const CodecArgs &a, uint32_t blocks, uint32_t wpb,
+                      hipStream_t stream) {
+  if (blocks == 0 || wpb == 0 || wpb > 4) return hipErrorInvalidValue;
+  const uint32_t lds = big_lds_bytes(wpb);
+  const void *fn = decode ? (const void *)bigunit_kernel<true> : (const void *)bigunit_kernel<false>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  if (decode)
+    hipLaunchKernelGGL((bigunit_kernel<true>), dim3(blocks), dim3(64 * wpb), lds, stream, a);
+  else
+    hipLaunchKernelGGL((bigunit_kernel<false>), dim3(blocks), dim3(64 * wpb), lds, stream, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_read_headers(const uint64_t *src, const uint64_t *len, uint32_t n, uint8_t *hdr,

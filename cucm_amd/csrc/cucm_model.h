@@ -10,7 +10,8 @@ namespace cucm {
 
 constexpr int kOrders = 8;        // context orders 0..7 (cucm_compress_v2.cu:552)
 constexpr int kMixCtx = 80;       // mixer weight sets (cucm_compress_v2.cu:554)
-constexpr int kMaxUnitBits = 256; // largest unit the LDS staging is laid out for
+constexpr int kMaxUnitBits = 256; // largest unit the main kernels' LDS staging is laid out for
+constexpr uint32_t kMaxBigUnitBits = 65504; // largest unit the container's U field holds (x 32)
 constexpr uint32_t kTeamWaves = 4; // waves of a team workgroup (one chunk per workgroup)
 constexpr uint32_t kFnvBasis = 2166136261u;
 constexpr uint32_t kFnvPrime = 16777619u;
@@ -87,6 +88,14 @@ struct Task {
 constexpr uint8_t kTaskTailPad = 1;
 constexpr uint8_t kTaskInitModel = 2;
 constexpr uint8_t kTaskFrozen = 4;  // frozen model: run by frozen_kernel, skipped by the others
+constexpr uint8_t kTaskBigUnit = 8; // unit above kMaxUnitBits: run by bigunit_kernel only
+
+// Bytes of unit records one wave of the large-unit codec keeps in global
+// memory for units of up to Umax bits: 8 slots (u32) and pr << 1 | y (u16)
+// per unit bit.
+constexpr uint64_t big_record_bytes(uint32_t Umax) {
+  return ((uint64_t)Umax * (8 * 4 + 2) + 255) & ~255ull;
+}
 
 // A trained initial model (prelearn) in device memory.  Order-0 states are
 // given by h0 (1..255) since order 0 has exactly 255 distinct slots.
@@ -117,6 +126,10 @@ struct CodecArgs {
   uint64_t *prof;          // optional per-phase cycle totals (kPhases), else null
   const DevTables *tabs;
   const DevInitModel *init;
+  uint8_t *big;            // large-unit codec: per-wave unit records
+  uint64_t big_bytes;      // bytes of records per wave
+  uint32_t big_umax;       // unit bits the records are laid out for
+  uint32_t pad;
 };
 
 // Container assembly for one job (file) of an encode launch.
@@ -159,6 +172,11 @@ hipError_t launch_codec(bool decode, const CodecArgs &a, uint32_t Umax, uint32_t
 uint32_t frozen_lds_bytes(uint32_t waves);
 hipError_t launch_frozen(bool decode, const CodecArgs &a, uint32_t blocks,
                          uint32_t waves_per_block, hipStream_t stream);
+// The large-unit codec (one wave per chunk of a task with kTaskBigUnit; the
+// others are skipped): LDS per workgroup of `waves` waves, and the launcher.
+uint32_t big_lds_bytes(uint32_t waves);
+hipError_t launch_big(bool decode, const CodecArgs &a, uint32_t blocks, uint32_t waves_per_block,
+                      hipStream_t stream);
 hipError_t launch_read_headers(const uint64_t *src, const uint64_t *len, uint32_t n, uint8_t *hdr,
                                hipStream_t stream);
 hipError_t launch_build_decode_tasks(const ParseJob *jobs, uint32_t n, Task *tasks,

@@ -116,6 +116,8 @@ extern "C" int cucm_compress_device_noalloc(cucm_ctx *ctx, const uint8_t *d_in, 
   int rc = validate_params(*p, in_len);
   if (rc) return rc;
   if (p->prelearn_size) return CUCM_E_UNSUPPORTED;  // a trained model needs device memory
+  // units above 256 bits keep their records in per-wave global memory
+  if (p->unit_bits > kMaxUnitBits && !(p->flags & CUCM_FLAG_FROZEN)) return CUCM_E_UNSUPPORTED;
   if (ctx->device < 0 || ctx->device >= kMaxDevices) return CUCM_E_UNSUPPORTED;
   std::lock_guard<std::mutex> dl(g_dev_mu[ctx->device]);
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -161,13 +163,16 @@ extern "C" int cucm_compress_device_noalloc(cucm_ctx *ctx, const uint8_t *d_in, 
   const uint64_t slot_bytes = frozen ? 0 : up256(7ull << p->table_bits);
   uint8_t *pool = base + off;
   const uint64_t pool_bytes = kNoAllocScratchBytes - off;
-  Shape sh, fz;
+  PassPlan pl;
+  pl.Umax = frozen ? 32u : p->unit_bits;
+  pl.slot_bytes = slot_bytes;
   if (frozen) {
-    fz = frozen_shape(ctx, C);
+    pl.fz = frozen_shape(ctx, C);
   } else {
-    sh = pick_shape(ctx, C, p->unit_bits, 0, false, slot_bytes, pool_bytes);
-    if (sh.slots() == 0) return CUCM_E_BUDGET;
+    pl.sh = pick_shape(ctx, C, p->unit_bits, 0, false, slot_bytes, pool_bytes);
+    if (pl.sh.slots() == 0) return CUCM_E_BUDGET;
   }
+  const Shape &sh = pl.sh, &fz = pl.fz;
   if (sh.blocks == 0 && fz.blocks == 0) return CUCM_E_BUDGET;
   hipLaunchKernelGGL(noalloc_tasks_kernel, dim3((C + 255) / 256), dim3(256), 0, s, d_in, N,
                      (uint32_t)S, (uint32_t)p->unit_bits, (uint32_t)p->table_bits, C,
@@ -185,7 +190,7 @@ extern "C" int cucm_compress_device_noalloc(cucm_ctx *ctx, const uint8_t *d_in, 
   a.tabs = (const DevTables *)ctx->tabs.p;
   a.init = nullptr;  // P = 0: no task carries kTaskInitModel
   cucm_stats st{};
-  rc = launch_pass(ctx, a, false, p->unit_bits, sh, fz, true, st);
+  rc = launch_pass(ctx, a, false, pl, true, st);
   if (rc) return rc;
   hipLaunchKernelGGL(noalloc_size_kernel, dim3(1), dim3(64), 0, s, results, C,
                      (uint32_t)p->batch_chunks, (uint64_t)21, info);

@@ -127,7 +127,7 @@ int cucm_stream_decompress_begin(cucm_ctx *ctx, const uint8_t *head, uint64_t he
   if (P > N || hp.table_bits < 8 || hp.table_bits > 25) return CUCM_E_CORRUPT;
   if (N - P > 0 && (hp.chunk_size == 0 || hp.unit_bits == 0 || hp.batch_chunks == 0))
     return CUCM_E_CORRUPT;
-  if (hp.unit_bits > kMaxUnitBits) return CUCM_E_UNSUPPORTED;
+  if (hp.unit_bits > kMaxBigUnitBits) return CUCM_E_CORRUPT;
   if (P > prelearn_cap || (P && !prelearn_out)) return CUCM_E_SMALL;
   auto s = std::make_unique<cucm_stream>();
   s->ctx = ctx;

@@ -52,8 +52,10 @@ extern "C" {
 
 typedef struct cucm_params {
   uint32_t chunk_size;    /* S, bytes per chunk (reference -s, default 1 MiB) */
-  uint16_t unit_bits;     /* U, bits per frozen-model unit, multiple of 32,
-                             32..256 in this build (reference -u, default 128) */
+  uint16_t unit_bits;     /* U, bits per frozen-model unit, a multiple of 32 up
+                             to 65,504 (reference -u, default 128); units above
+                             256 bits run on a slower single-wave codec that
+                             keeps the unit's records in device memory */
   uint16_t batch_chunks;  /* chunks per batch record (reference -b, default 100) */
   uint32_t prelearn_size; /* P, leading bytes that train the initial model (-p) */
   uint8_t table_bits;     /* log2 slots per order table, 8..25; 25 = reference */

@@ -192,3 +192,41 @@ def test_emulated_frozen_codec_matches_oracle(emu, oracle, tmp_path, kind, N, S,
                        capture_output=True, text=True, env=env)
     assert r.returncode == 0, r.stderr[-4000:]
     assert dec.read_bytes() == data
+
+
+BIG_CASES = [
+    # (kind, N, S, U, tb, tail_pad): units above the main kernels' 256 bits
+    ("markov2", 3000, 1024, 288, 12, True),
+    ("mixed", 5000, 2048, 512, 14, True),
+    ("structured", 2100, 4096, 1024, 10, True),
+    ("markov2", 9000, 4096, 4096, 12, True),
+    ("zeros", 3000, 1024, 320, 8, True),
+    ("runs", 4000, 2048, 2048, 9, False),
+    ("random", 40, 4096, 65504, 8, True),
+]
+
+
+@pytest.mark.parametrize("kind,N,S,U,tb,pad", BIG_CASES)
+def test_emulated_large_unit_codec_matches_oracle(emu, oracle, tmp_path, kind, N, S, U, tb, pad):
+    """The large-unit codec (cucm_bigunit.h, U > 256) on one emulated wave
+    whose model slot and unit records persist across chunks: chunk payloads
+    equal to the oracle's (tail padding included), decodes lossless; zeros,
+    runs and 2^8-slot tables make slots repeat inside an update block."""
+    data = _data(kind, N)
+    inp, enc, dec = tmp_path / "in", tmp_path / "enc", tmp_path / "dec"
+    inp.write_bytes(data)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([emu, "benc", str(inp), str(S), str(U), str(tb), "1" if pad else "0",
+                        str(enc)], capture_output=True, text=True, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
+    pays = _payloads(enc.read_bytes())
+    C = (N + S - 1) // S
+    assert len(pays) == C
+    for c in range(C):
+        ref = oracle.encode_chunk(data[c * S:(c + 1) * S], S, U, oracle.Model(tb),
+                                  tail_pad=pad and c == C - 1)
+        assert pays[c] == ref, "chunk %d" % c
+    r = subprocess.run([emu, "bdec", str(enc), str(N), str(S), str(U), str(tb), str(dec)],
+                       capture_output=True, text=True, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert dec.read_bytes() == data

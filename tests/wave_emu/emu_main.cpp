@@ -18,9 +18,15 @@
 //       wave, with the zero model or the model trained on the prelearn file
 //   wave_emu fdec <payloads> <N> <S> <tb> <spec_bits> <prelearn|-> <output>
 //       the same through the frozen-model decoder
+//   wave_emu benc <input> <S> <U> <tb> <tail_pad> <output>
+//       enc through the large-unit codec (cucm_bigunit.h), any U, one wave
+//       whose model slot and unit records persist across chunks
+//   wave_emu bdec <payloads> <N> <S> <U> <tb> <output>
+//       the same through the large-unit decoder
 #define CUCM_WAVE_EMULATION 1
 #include "wave_emulation.h"
 // (the device code follows the emulation shims)
+#include "cucm_bigunit.h"
 #include "cucm_frozen.h"
 #include "cucm_team.h"
 #include "cucm_host_model.h"
@@ -404,6 +410,85 @@ static void frozen_decode_all(const std::vector<uint8_t> &in, uint8_t *out, uint
   free(base);
 }
 
+// Every chunk through one large-unit wave (exact-size LDS, slot and records).
+static std::vector<uint8_t> big_run(bool decode, const std::vector<uint8_t> &in, uint64_t N,
+                                    uint32_t S, uint32_t U, uint32_t tb, bool pad) {
+  const size_t tab = align16(sizeof(DevTables));
+  const size_t bytes = tab + big_wave_bytes();
+  uint8_t *base = (uint8_t *)aligned_alloc(16, (bytes + 15) & ~(size_t)15);
+  memset(base, 0xa5, bytes);
+  memcpy(base, &host_tables(), sizeof(DevTables));
+  const DevTables *T = (const DevTables *)base;
+  const BigMem M = carve_big(base + tab);
+  uint8_t *slot = (uint8_t *)malloc((size_t)7 << tb);
+  const uint64_t rb = big_record_bytes(U);
+  uint8_t *recs = (uint8_t *)malloc(rb);
+  memset(recs, 0x5a, rb);
+  const BigRec R = big_records(recs, U);
+  std::vector<uint8_t> out;
+  if (decode) out.resize(N);
+  const uint64_t C = (N + S - 1) / S;
+  size_t pos = 0;
+  for (uint64_t c = 0; c < C; ++c) {
+    const uint32_t L = (uint32_t)std::min<uint64_t>(S, N - c * S);
+    Task tk{};
+    std::vector<uint8_t> buf;
+    uint8_t *dstb;
+    uint32_t cap = 0;
+    if (decode) {
+      uint32_t n = 0;
+      for (int i = 0; i < 4; ++i) n |= (uint32_t)in[pos + i] << (8 * i);
+      pos += 4;
+      buf.assign(in.begin() + pos, in.begin() + pos + n);
+      pos += n;
+      dstb = (uint8_t *)malloc(L);
+      tk.aux = n;
+    } else {
+      buf.assign(in.begin() + c * S, in.begin() + c * S + L);
+      cap = align16(S + S / 2 + 256);
+      dstb = (uint8_t *)malloc(cap);
+      tk.aux = cap;
+    }
+    uint8_t *src = (uint8_t *)malloc(buf.size() ? buf.size() : 1);
+    memcpy(src, buf.data(), buf.size());
+    tk.src = (uint64_t)(uintptr_t)src;
+    tk.dst = (uint64_t)(uintptr_t)dstb;
+    tk.L = L;
+    tk.S = S;
+    tk.U = (uint16_t)U;
+    tk.tb = (uint8_t)tb;
+    tk.flags = (uint8_t)(kTaskBigUnit | ((pad && c == C - 1) ? kTaskTailPad : 0));
+    uint64_t result = 0;
+    CodecArgs a{};
+    a.tasks = &tk;
+    a.ntasks = 1;
+    a.result = &result;
+    emu::run_wave([&](uint32_t lane) {
+      if (decode)
+        big_decode_task(a, tk, 0, M, T, slot, R, lane);
+      else
+        big_encode_task(a, tk, 0, M, T, slot, R, lane);
+    });
+    if (decode) {
+      memcpy(out.data() + c * S, dstb, L);
+    } else {
+      if (result & kResultOverflow) {
+        fprintf(stderr, "staging overflow\n");
+        exit(3);
+      }
+      const uint32_t n = (uint32_t)result;
+      for (int i = 0; i < 4; ++i) out.push_back((uint8_t)(n >> (8 * i)));
+      out.insert(out.end(), dstb, dstb + n);
+    }
+    free(src);
+    free(dstb);
+  }
+  free(recs);
+  free(slot);
+  free(base);
+  return out;
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) return 2;
   const std::string mode = argv[1];
@@ -514,6 +599,19 @@ int main(int argc, char **argv) {
     write_file(argv[8], out);
     return 0;
   }
-  fprintf(stderr, "usage: wave_emu enc|dec|tenc|tdec|fenc|fdec ...\n");
+  if (mode == "benc" && argc == 8) {
+    const std::vector<uint8_t> in = read_file(argv[2]);
+    const uint32_t S = atoi(argv[3]), U = atoi(argv[4]), tb = atoi(argv[5]);
+    write_file(argv[7], big_run(false, in, in.size(), S, U, tb, atoi(argv[6]) != 0));
+    return 0;
+  }
+  if (mode == "bdec" && argc == 8) {
+    const std::vector<uint8_t> in = read_file(argv[2]);
+    const uint64_t N = strtoull(argv[3], nullptr, 10);
+    const uint32_t S = atoi(argv[4]), U = atoi(argv[5]), tb = atoi(argv[6]);
+    write_file(argv[7], big_run(true, in, N, S, U, tb, false));
+    return 0;
+  }
+  fprintf(stderr, "usage: wave_emu enc|dec|tenc|tdec|fenc|fdec|benc|bdec ...\n");
   return 2;
 }
