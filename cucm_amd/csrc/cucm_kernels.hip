@@ -157,6 +157,175 @@ __global__ void __launch_bounds__(256) bigunit_kernel(CodecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Serving: one persistent launch over a ring of chunk tasks that the host
+// keeps appending to (tail, in host memory), encoder and decoder tasks of
+// units up to 128 bits mixed.  Requests' inputs, containers, decoded bytes
+// and prelearned models live in host memory shared with the device, so the
+// kernel reads nothing the host wrote into device memory while it runs.
+// The wave that finishes a job's last task assembles its container (compress)
+// and raises the job's done word.
+// ---------------------------------------------------------------------------
+
+constexpr int kServeNJ = 2;  // units up to 128 bits
+constexpr int kServeD = 2;   // decoder: 2-bit speculative rounds
+
+// n bytes from s to d, the wave's lanes striding: head bytes until d is
+// 4-aligned, then words assembled from s, then the tail bytes.
+__device__ __forceinline__ void wave_copy(uint8_t *d, const uint8_t *s, uint64_t n,
+                                          uint32_t lane) {
+  const uint32_t head = (uint32_t)((4u - ((uintptr_t)d & 3u)) & 3u);
+  const uint64_t hn = head < n ? head : n;
+  if (lane < hn) d[lane] = s[lane];
+  const uint64_t nwords = (n - hn) >> 2;
+  uint32_t *dw = (uint32_t *)(d + hn);
+  const uint8_t *sb = s + hn;
+  for (uint64_t i = lane; i < nwords; i += 64) {
+    const uint8_t *p = sb + 4 * i;
+    dw[i] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+            ((uint32_t)p[3] << 24);
+  }
+  const uint64_t t0 = hn + 4 * nwords;
+  if (lane < n - t0) d[t0 + lane] = s[t0 + lane];
+}
+
+// A compress job's batch records after its header: per batch the LE blob
+// length, the BE end offsets E_c = 4C + sum_{j<=c} size_j - 1 and the
+// payloads (cucm_compress_v2.cu:870-914, 1242-1244).  Returns the container
+// length, or 0 if a chunk overflowed its staging region.
+__device__ uint64_t serve_assemble(const ServeJob &J, const Task *ring, const uint64_t *res,
+                                   uint32_t mask, uint32_t lane) {
+  uint8_t *o = (uint8_t *)(uintptr_t)J.out_base;
+  uint64_t pos = J.header_len;
+  uint32_t bad = 0;
+  for (uint32_t c0 = 0; c0 < J.ntasks; c0 += J.batch) {
+    const uint32_t C = min(J.batch, J.ntasks - c0);
+    uint64_t e = 4ull * C;  // running end of the blob
+    for (uint32_t g = 0; g < C; g += 64) {
+      const uint32_t c = g + lane;
+      const uint64_t r = c < C ? res[(J.first + c0 + c) & mask] : 0ull;
+      bad |= __ballot((r & kResultOverflow) != 0) != 0ull ? 1u : 0u;
+      const uint32_t sz = (uint32_t)(r & ~kResultOverflow);
+      uint64_t run = 0, mine = 0;
+      for (uint32_t k = 0; k < 64; ++k) {
+        run += __builtin_amdgcn_readlane(sz, k);
+        mine = k == lane ? run : mine;
+      }
+      if (c < C) {
+        const uint64_t E = e + mine - 1;
+        uint8_t *q = o + pos + 4 + 4ull * c;
+        q[0] = (uint8_t)(E >> 24);
+        q[1] = (uint8_t)(E >> 16);
+        q[2] = (uint8_t)(E >> 8);
+        q[3] = (uint8_t)E;
+      }
+      e += run;
+    }
+    if (lane == 0) {
+      o[pos + 0] = (uint8_t)e;
+      o[pos + 1] = (uint8_t)(e >> 8);
+      o[pos + 2] = (uint8_t)(e >> 16);
+      o[pos + 3] = (uint8_t)(e >> 24);
+    }
+    uint64_t off = 4ull * C;
+    for (uint32_t c = 0; c < C; ++c) {
+      const uint32_t ri = (J.first + c0 + c) & mask;
+      const uint64_t sz = res[ri] & ~kResultOverflow;
+      const uint8_t *src = (const uint8_t *)(uintptr_t)ring[ri].dst;
+      if (!bad) wave_copy(o + pos + 4 + off, src, sz, lane);
+      off += sz;
+    }
+    pos += 4 + e;
+  }
+  return bad ? 0ull : pos;
+}
+
+__global__ void __launch_bounds__(512) serve_kernel(ServeArgs s) {
+  constexpr int NJ = kServeNJ, D = kServeD;
+  constexpr uint32_t Umax = 64u * NJ, P = (1u << D) - 1u;
+  extern __shared__ __align__(16) uint8_t lds[];
+  LDS DevTables *T = (LDS DevTables *)lds;
+  stage_tables(s.a.tabs, T);
+  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t wbytes = wave_lds_bytes(Umax, P, true);
+  const WaveMem M = carve((LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * wbytes, Umax, P,
+                          true);
+  const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wid;
+  GLB uint8_t *slot = (GLB uint8_t *)s.a.pool + gw * s.a.slot_bytes;
+  uint64_t idle0 = __builtin_amdgcn_s_memrealtime();
+  uint32_t spins = 0;
+  for (;;) {
+    // claim the next published task, if any (compare-and-swap on the head,
+    // so a wave that leaves never holds an unpublished index).  The poll of
+    // the tail is a relaxed load: an acquire at system scope invalidates the
+    // L2, and idle waves polling that way starve the waves that code.
+    uint32_t t = 0, got = 0;
+    if (lane == 0) {
+      const uint32_t tl = __hip_atomic_load(s.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      uint32_t h = __hip_atomic_load(s.a.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      while ((int32_t)(tl - h) > 0) {
+        const uint32_t old = atomicCAS(s.a.counter, h, h + 1u);
+        if (old == h) {
+          got = 1u;
+          t = h;
+          break;
+        }
+        h = old;
+      }
+    }
+    got = __builtin_amdgcn_readfirstlane(got);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (!got) {
+      uint32_t stop = 0;
+      if (lane == 0) stop = __hip_atomic_load(s.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      stop = __builtin_amdgcn_readfirstlane(stop);
+      const uint64_t now = __builtin_amdgcn_s_memrealtime();
+      if (stop || now - idle0 > s.idle_ticks) break;
+      // poll often just after a task, then every ~25 us (the tail lives in
+      // host memory: an idle wave's poll is a read across the host link)
+      if (++spins < 64) {
+        __builtin_amdgcn_s_sleep(32);
+      } else {
+        for (int k = 0; k < 8; ++k) __builtin_amdgcn_s_sleep(127);
+      }
+      continue;
+    }
+    spins = 0;
+    // with the relaxed load of the tail above, synchronises with the host's
+    // release of the tail: the task's host data is read after it
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+    const uint32_t ri = t & s.ring_mask;
+    const Task tk = s.a.tasks[ri];
+    if (tk.flags & kTaskDecode)
+      decode_task<NJ, D, false>(s.a, tk, ri, M, T, slot, Umax, lane);
+    else
+      encode_task<NJ, false>(s.a, tk, ri, M, T, slot, Umax, lane);
+    // this task's payload and result word are visible before it is counted
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    uint32_t prev = 0;
+    if (lane == 0) prev = atomicAdd(&s.jobcount[tk.job], 1u);
+    prev = __builtin_amdgcn_readfirstlane(prev);
+    const ServeJob J = s.jobs[tk.job];
+    if (prev + 1u == J.ntasks) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every task of the job is done
+      uint64_t len = 0;
+      if (!(tk.flags & kTaskDecode)) len = serve_assemble(J, s.a.tasks, s.a.result, s.ring_mask, lane);
+      if (lane == 0) __hip_atomic_store(&s.jobcount[tk.job], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // the container (or the decoded bytes of every task) reaches the host
+      // before the done word
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+      if (lane == 0) s.lens[tk.job] = len;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+      if (lane == 0) {
+        const bool failed = !(tk.flags & kTaskDecode) && len == 0;
+        __hip_atomic_store(&s.done[tk.job], J.id | (failed ? 0x80000000u : 0u), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
+    idle0 = __builtin_amdgcn_s_memrealtime();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Container assembly: per job, per batch: LE u32 blob length, BE end
 // offsets E_c = 4C + sum_{j<=c} size_j - 1, payloads back to back
 // (cucm_compress_v2.cu:870-914, 1242-1244).
@@ -293,7 +462,7 @@ __global__ void build_decode_tasks_kernel(const ParseJob *jobs, uint32_t n, Task
       t.tb = (uint8_t)J.tb;
       t.flags = (uint8_t)J.flags;
       t.model = J.model;
-      t.pad = 0;
+      t.job = 0;
       tasks[J.first_task + ci] = t;
       prev = E;
     }
@@ -461,6 +630,21 @@ hipError_t launch_frozen(bool decode, const CodecArgs &a, uint32_t blocks, uint3
     case 8: return launch_frozen_one<true, 8>(a, blocks, wpb, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+uint32_t serve_lds_bytes(uint32_t waves) {
+  return align16(sizeof(DevTables)) +
+         waves * wave_lds_bytes(64u * kServeNJ, (1u << kServeD) - 1u, true);
+}
+
+hipError_t launch_serve(const ServeArgs &s, uint32_t blocks, uint32_t wpb, hipStream_t stream) {
+  if (blocks == 0 || wpb == 0 || wpb > 8) return hipErrorInvalidValue;
+  const uint32_t lds = serve_lds_bytes(wpb);
+  hipError_t e = hipFuncSetAttribute((const void *)serve_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(serve_kernel, dim3(blocks), dim3(64 * wpb), lds, stream, s);
+  return hipGetLastError();
 }
 
 uint32_t big_lds_bytes(uint32_t waves) {

@@ -83,12 +83,13 @@ struct Task {
   uint8_t tb;      // log2 slots per order table
   uint8_t flags;   // kTaskTailPad | kTaskInitModel | kTaskFrozen
   uint32_t model;  // index of the initial model (kTaskInitModel)
-  uint32_t pad;
+  uint32_t job;    // serving kernel: the task's job slot (else 0)
 };
 constexpr uint8_t kTaskTailPad = 1;
 constexpr uint8_t kTaskInitModel = 2;
 constexpr uint8_t kTaskFrozen = 4;  // frozen model: run by frozen_kernel, skipped by the others
 constexpr uint8_t kTaskBigUnit = 8; // unit above kMaxUnitBits: run by bigunit_kernel only
+constexpr uint8_t kTaskDecode = 16; // serving kernel: a decoder task (else an encoder task)
 
 // Bytes of unit records one wave of the large-unit codec keeps in global
 // memory for units of up to Umax bits: 8 slots (u32) and pr << 1 | y (u16)
@@ -155,6 +156,36 @@ struct ParseJob {
   uint32_t S, U, tb, batch, flags, model, pad;
 };
 
+// A job of the serving kernel, in host memory shared with the device,
+// written by the host before the job's first task is published.
+struct ServeJob {
+  uint64_t out_base;    // compress: the container in host memory (header already there)
+  uint32_t first;       // ring index of the job's first task
+  uint32_t ntasks;
+  uint32_t header_len;  // compress: header + prelearn segment bytes at out_base
+  uint32_t batch;       // compress: chunks per batch record
+  uint32_t id;          // written to the job's done word when the job is complete
+  uint32_t pad;
+};
+
+// Arguments of the serving kernel: persistent waves take tasks from a ring
+// in host memory as the host publishes them (tail), until the host sets the
+// stop word (the ring is drained first) or, as a safeguard, no task came for
+// idle_ticks of the 100 MHz real-time clock.
+struct ServeArgs {
+  CodecArgs a;                 // tasks = the ring, result = per-ring-slot words,
+                               // counter = the ring's claim head (device memory)
+  const uint32_t *tail;        // host memory: tasks published so far (wrapping)
+  const uint32_t *stop;        // host memory: nonzero ends the kernel
+  const ServeJob *jobs;        // host memory: [job_mask + 1]
+  uint32_t *jobcount;          // device memory: finished tasks per job slot
+  uint32_t *done;              // host memory: per job slot, the id of its last completed job
+                               // (bit 31 set: a chunk overflowed its staging region)
+  uint64_t *lens;              // host memory: per job slot, the container length (compress)
+  uint32_t ring_mask, job_mask;
+  uint64_t idle_ticks;
+};
+
 }  // namespace cucm
 
 #ifdef CUCM_WITH_LAUNCHERS
@@ -177,6 +208,12 @@ hipError_t launch_frozen(bool decode, const CodecArgs &a, uint32_t blocks,
 uint32_t big_lds_bytes(uint32_t waves);
 hipError_t launch_big(bool decode, const CodecArgs &a, uint32_t blocks, uint32_t waves_per_block,
                       hipStream_t stream);
+// The serving kernel (one wave per chunk task, encoder and decoder tasks of
+// units up to 128 bits in one launch, 2-bit speculative decoding): LDS per
+// workgroup of `waves` waves, and the launcher.
+uint32_t serve_lds_bytes(uint32_t waves);
+hipError_t launch_serve(const ServeArgs &s, uint32_t blocks, uint32_t waves_per_block,
+                        hipStream_t stream);
 hipError_t launch_read_headers(const uint64_t *src, const uint64_t *len, uint32_t n, uint8_t *hdr,
                                hipStream_t stream);
 hipError_t launch_build_decode_tasks(const ParseJob *jobs, uint32_t n, Task *tasks,

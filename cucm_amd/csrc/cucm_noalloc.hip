@@ -56,7 +56,7 @@ __global__ void noalloc_tasks_kernel(const uint8_t *in, uint64_t n, uint32_t S, 
   t.tb = (uint8_t)tb;
   t.flags = (uint8_t)(((pad && c == C - 1) ? kTaskTailPad : 0) | (frozen ? kTaskFrozen : 0));
   t.model = 0;
-  t.pad = 0;
+  t.job = 0;
   tasks[c] = t;
 }
 

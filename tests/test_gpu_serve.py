@@ -133,3 +133,40 @@ def test_serve_poisson_mix_of_the_serving_configuration(ctx):
         sc.serve_stop()
     finally:
         sc.close()
+
+
+def test_requests_admitted_into_the_running_kernel(ctx):
+    """A large compress request is submitted first; small requests submitted
+    while it runs start on free waves of the same launch and finish before
+    it (they do not wait for it), and every result equals the request run
+    alone."""
+    p = make_params(16384, 128, 100, 0, 18)
+    big = datagen.markov2(24 << 20, 61).tobytes()
+    smalls = [datagen.markov2(20000 + 997 * i, 70 + i).tobytes() for i in range(12)]
+    big_alone = ctx.compress(big, p)
+    small_alone = [ctx.compress(s, p) for s in smalls]
+    sc = cucm_amd.Context()
+    try:
+        sc.serve_start()
+        launches0 = sc.serve_launches()
+        jb = sc.submit_compress(big, p)
+        assert sc.poll(jb, wait=False) is None  # still running
+        ids = [sc.submit_compress(s, p) for s in smalls]
+        done_small = []
+        for i, jid in enumerate(ids):
+            out, info = sc.poll(jid, wait=True)
+            assert out == small_alone[i], "small request %d" % i
+            assert info["batch_jobs"] >= 2  # admitted while the large one was in flight
+            done_small.append(info["latency_ms"])
+        assert sc.poll(jb, wait=False) is None, "the small requests waited for the large one"
+        out, info = sc.poll(jb, wait=True)
+        assert out == big_alone
+        assert max(done_small) < info["latency_ms"]
+        assert sc.serve_launches() - launches0 <= 2  # one running launch took them all
+        # decompressions of the results through the same kernel
+        ids = [sc.submit_decompress(b) for b in [big_alone] + small_alone]
+        for jid, want in zip(ids, [big] + smalls):
+            assert sc.poll(jid, wait=True)[0] == want
+        sc.serve_stop()
+    finally:
+        sc.close()
