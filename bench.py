@@ -319,6 +319,8 @@ def main():
     ap.add_argument("--cpu-mib", type=int, default=12,
                     help="batch workload: MiB of files in the all-thread CPU baseline sample")
     ap.add_argument("--requests", type=int, default=240, help="serve workload: requests per run")
+    ap.add_argument("--serve-requests", type=int, default=240,
+                    help="batch workload: requests of the serving mix measured beside it (0 = none)")
     ap.add_argument("--lanes", type=int, default=0, help="serve workload: concurrent lanes (0 = 3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

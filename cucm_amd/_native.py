@@ -25,7 +25,7 @@ EXPORTS = [
     "cucm_ctx_malloc", "cucm_ctx_free", "cucm_ctx_memcpy_h2d", "cucm_ctx_memcpy_d2h",
     "cucm_ctx_synchronize", "cucm_ctx_last_stats", "cucm_ctx_set_max_waves",
     "cucm_ctx_set_phase_profile", "cucm_ctx_phase_cycles",
-    "cucm_serve_start", "cucm_serve_stop", "cucm_submit", "cucm_poll", "cucm_serve_launches",
+    "cucm_serve_start", "cucm_serve_start_ex", "cucm_serve_stop", "cucm_submit", "cucm_poll", "cucm_serve_launches",
     "cucm_ctx_set_waves_per_block", "cucm_ctx_set_team", "cucm_container_head_len", "cucm_stream_compress_begin", "cucm_stream_compress_next",
     "cucm_stream_decompress_begin", "cucm_stream_decompress_next", "cucm_stream_end",
     "cucm_compress_device_noalloc",
@@ -140,6 +140,7 @@ def lib():
         L.cucm_ctx_set_phase_profile.argtypes = [vp, ctypes.c_int]
         L.cucm_ctx_phase_cycles.argtypes = [vp, P(u64)]
         L.cucm_serve_start.argtypes = [vp, u32, u32]
+        L.cucm_serve_start_ex.argtypes = [vp, u32, u32, u32]
         L.cucm_serve_launches.argtypes = [vp, P(u64)]
         L.cucm_serve_stop.argtypes = [vp]
         L.cucm_submit.argtypes = [vp, P(Job), P(u64)]

@@ -171,9 +171,11 @@ def autotune_batch(ctx, arena, offs, sizes, sample_bytes=640 << 20, ratio_tol=0.
 
 
 def apply_launch(ctx, launch):
-    """Sets a direction's launch settings (team, waves per workgroup) on ctx."""
+    """Sets a direction's launch settings (team: True always, False never,
+    None automatic; waves per workgroup, 0 automatic) on ctx."""
     ctx.set_waves_per_block(launch.get("waves_per_block", 0))
-    ctx.set_team(2 if launch.get("team") else 1)
+    team = launch.get("team")
+    ctx.set_team(0 if team is None else (2 if team else 1))
 
 
 def save_settings(res, path=SETTINGS, note=""):

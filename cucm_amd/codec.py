@@ -354,9 +354,13 @@ class Context:
         return b"".join(parts)
 
     # ---- serving loop (continuous batching) ------------------------------
-    def serve_start(self, lanes=0, max_inflight=0):
-        """Starts the request loop: `lanes` concurrent launches (0 = 3)."""
-        check(lib().cucm_serve_start(self.handle, lanes, max_inflight), "serve_start")
+    def serve_start(self, lanes=0, max_inflight=0, static_lanes=False):
+        """Starts the request loop: the serving kernel takes requests as job
+        slots (`max_inflight`, 0 = 256) free up, with `lanes` static-batch
+        lanes for the requests it does not take; `static_lanes` runs every
+        request on the lanes instead (0 = 3 lanes)."""
+        check(lib().cucm_serve_start_ex(self.handle, lanes, max_inflight, 1 if static_lanes else 0),
+              "serve_start")
         self._inflight = {}
 
     def serve_stop(self):

@@ -251,16 +251,17 @@ __global__ void __launch_bounds__(512) serve_kernel(ServeArgs s) {
                           true);
   const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wid;
   GLB uint8_t *slot = (GLB uint8_t *)s.a.pool + gw * s.a.slot_bytes;
+  uint32_t *mtail = s.mirror, *mstop = s.mirror + 1;
+  uint64_t *mtime = (uint64_t *)(s.mirror + 2);
   uint64_t idle0 = __builtin_amdgcn_s_memrealtime();
   uint32_t spins = 0;
   for (;;) {
     // claim the next published task, if any (compare-and-swap on the head,
-    // so a wave that leaves never holds an unpublished index).  The poll of
-    // the tail is a relaxed load: an acquire at system scope invalidates the
-    // L2, and idle waves polling that way starve the waves that code.
+    // so a wave that leaves never holds an unpublished index), against the
+    // device mirror of the tail
     uint32_t t = 0, got = 0;
     if (lane == 0) {
-      const uint32_t tl = __hip_atomic_load(s.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      const uint32_t tl = __hip_atomic_load(mtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t h = __hip_atomic_load(s.a.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       while ((int32_t)(tl - h) > 0) {
         const uint32_t old = atomicCAS(s.a.counter, h, h + 1u);
@@ -275,13 +276,38 @@ __global__ void __launch_bounds__(512) serve_kernel(ServeArgs s) {
     got = __builtin_amdgcn_readfirstlane(got);
     t = __builtin_amdgcn_readfirstlane(t);
     if (!got) {
+      // Refresh the mirror from host memory: one wave at a time, at most once
+      // per poll_ticks.  Host words are read across the host link, and idle
+      // waves reading them all the time delay the working waves' own host
+      // traffic many times over; an acquire at system scope on every poll
+      // would also invalidate the L2 under the working waves.
       uint32_t stop = 0;
-      if (lane == 0) stop = __hip_atomic_load(s.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      stop = __builtin_amdgcn_readfirstlane(stop);
       const uint64_t now = __builtin_amdgcn_s_memrealtime();
+      if (lane == 0) {
+        const uint64_t last = __hip_atomic_load(mtime, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (now - last >= s.poll_ticks && atomicCAS((unsigned long long *)mtime, last, now) == last) {
+          const uint32_t ht = __hip_atomic_load(s.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          const uint32_t hs = __hip_atomic_load(s.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          uint32_t cur = __hip_atomic_load(mtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((int32_t)(ht - cur) > 0) {
+            // synchronises with the host's release of the tail, and passes
+            // that on to the wave that reads the mirror and claims a task
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            while ((int32_t)(ht - cur) > 0) {
+              const uint32_t o = atomicCAS(mtail, cur, ht);
+              if (o == cur) break;
+              cur = o;
+            }
+          }
+          if (hs) __hip_atomic_store(mstop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        stop = __hip_atomic_load(mstop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      stop = __builtin_amdgcn_readfirstlane(stop);
       if (stop || now - idle0 > s.idle_ticks) break;
-      // poll often just after a task, then every ~25 us (the tail lives in
-      // host memory: an idle wave's poll is a read across the host link)
+      // look often just after a task, then every ~25 us: hundreds of idle
+      // waves polling device memory faster slow the working waves' gathers
       if (++spins < 64) {
         __builtin_amdgcn_s_sleep(32);
       } else {
@@ -290,15 +316,23 @@ __global__ void __launch_bounds__(512) serve_kernel(ServeArgs s) {
       continue;
     }
     spins = 0;
-    // with the relaxed load of the tail above, synchronises with the host's
-    // release of the tail: the task's host data is read after it
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+    // with the mirror's release above: the task's host data is read after
+    // the host published it
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const uint32_t ri = t & s.ring_mask;
     const Task tk = s.a.tasks[ri];
-    if (tk.flags & kTaskDecode)
+    if (tk.flags & kTaskDecode) {
+      // decoded into device staging, then copied to the host buffer at once:
+      // a store to host memory per unit would hold up every later wait of
+      // the decoder on its gathers (stores count in the same counter)
       decode_task<NJ, D, false>(s.a, tk, ri, M, T, slot, Umax, lane);
-    else
+      const ServeJob Jd = s.jobs[tk.job];
+      uint8_t *hd = (uint8_t *)(uintptr_t)Jd.out_base + Jd.header_len +
+                    (uint64_t)((ri - Jd.first) & s.ring_mask) * tk.S;
+      wave_copy(hd, (const uint8_t *)(uintptr_t)tk.dst, tk.L, lane);
+    } else {
       encode_task<NJ, false>(s.a, tk, ri, M, T, slot, Umax, lane);
+    }
     // this task's payload and result word are visible before it is counted
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     uint32_t prev = 0;

@@ -159,10 +159,12 @@ struct ParseJob {
 // A job of the serving kernel, in host memory shared with the device,
 // written by the host before the job's first task is published.
 struct ServeJob {
-  uint64_t out_base;    // compress: the container in host memory (header already there)
+  uint64_t out_base;    // host memory: the container (compress; header already there)
+                        // or the decoded bytes (decompress; prelearn bytes already there)
   uint32_t first;       // ring index of the job's first task
   uint32_t ntasks;
-  uint32_t header_len;  // compress: header + prelearn segment bytes at out_base
+  uint32_t header_len;  // compress: header + prelearn segment bytes at out_base;
+                        // decompress: prelearn bytes (chunk c's bytes at + c * S)
   uint32_t batch;       // compress: chunks per batch record
   uint32_t id;          // written to the job's done word when the job is complete
   uint32_t pad;
@@ -171,7 +173,9 @@ struct ServeJob {
 // Arguments of the serving kernel: persistent waves take tasks from a ring
 // in host memory as the host publishes them (tail), until the host sets the
 // stop word (the ring is drained first) or, as a safeguard, no task came for
-// idle_ticks of the 100 MHz real-time clock.
+// idle_ticks of the 100 MHz real-time clock.  Idle waves watch a mirror of
+// the tail and stop words in device memory; one wave at a time refreshes it
+// from host memory.
 struct ServeArgs {
   CodecArgs a;                 // tasks = the ring, result = per-ring-slot words,
                                // counter = the ring's claim head (device memory)
@@ -182,8 +186,11 @@ struct ServeArgs {
   uint32_t *done;              // host memory: per job slot, the id of its last completed job
                                // (bit 31 set: a chunk overflowed its staging region)
   uint64_t *lens;              // host memory: per job slot, the container length (compress)
+  uint32_t *mirror;            // device memory: [0] the tail as last read from the host,
+                               // [1] the stop word likewise, [2..3] the time of that read
   uint32_t ring_mask, job_mask;
   uint64_t idle_ticks;
+  uint64_t poll_ticks;         // the host words are read at most once per poll_ticks
 };
 
 }  // namespace cucm

@@ -19,7 +19,10 @@
 // Requests the serving kernel does not take (frozen model, units above 128
 // bits, tables above 2^20 slots, anything larger than the arenas, or a chunk
 // that overflowed its staging region) go to fallback lanes that run them as
-// static batches through the one-launch-per-call path.
+// static batches through the one-launch-per-call path.  Started with
+// CUCM_SERVE_STATIC_LANES, every request takes the lanes (the design before
+// the serving kernel: each lane takes the oldest pending requests of one kind
+// and runs them as one batch to the end).
 //
 // Each request's output depends only on its own input and parameters, never
 // on what it ran beside.
@@ -47,6 +50,7 @@ constexpr uint32_t kServeTbMax = 20;      // table bits its model slots hold
 constexpr uint32_t kRing = 1u << 16;      // task ring entries
 constexpr uint64_t kIdleTicks = 200000000;  // a wave leaves after 2 s without a task
 constexpr uint64_t kStopAfterNs = 20000000;   // the kernel is stopped 20 ms after the last job
+constexpr uint64_t kPollTicks = 200;          // host words read at most every 2 us
 
 uint64_t clock_ns() {
   return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -213,6 +217,7 @@ struct Server {
       __atomic_store_n(&ctl->stop, 1u, __ATOMIC_RELEASE);
       if (ks) (void)hipStreamSynchronize(ks);
     }
+
     if (ks) (void)hipStreamDestroy(ks);
   }
 
@@ -254,7 +259,10 @@ int Server::init_engine(uint32_t max_jobs) {
   darena.init(arena);
   // the kernel's resident waves: as LDS allows, within the rest of the budget
   slot_bytes = up256(7ull << kServeTbMax);
-  uint32_t wmax = 8;
+  // workgroups of 4 waves, one per SIMD: a fifth wave on a CU shares a SIMD
+  // with an older wave and runs its task several times slower, and a
+  // request's latency is that of its slowest task
+  uint32_t wmax = 4;
   while (wmax > 1 && serve_lds_bytes(wmax) > c->lds_block) --wmax;
   const uint64_t per_cu = std::max<uint64_t>(1, c->lds_cu / serve_lds_bytes(wmax));
   uint64_t waves = (uint64_t)c->cus * per_cu * wmax;
@@ -266,9 +274,9 @@ int Server::init_engine(uint32_t max_jobs) {
   shape.wpb = wpb;
   shape.blocks = (uint32_t)std::max<uint64_t>(1, waves / wpb);
   if ((rc = pool.reserve(slot_bytes * shape.waves())) || (rc = results.reserve(8ull * kRing)) ||
-      (rc = head.reserve(16)) || (rc = jobcount.reserve(4ull * J)))
+      (rc = head.reserve(256)) || (rc = jobcount.reserve(4ull * J)))
     return rc;
-  if (hipMemset(head.p, 0, 16) != hipSuccess || hipMemset(jobcount.p, 0, 4ull * J) != hipSuccess)
+  if (hipMemset(head.p, 0, 256) != hipSuccess || hipMemset(jobcount.p, 0, 4ull * J) != hipSuccess)
     return CUCM_E_DEVICE;
   slot_job.assign(J, 0);
   for (uint32_t s = J; s-- > 0;) free_slots.push_back(s);
@@ -421,7 +429,8 @@ int Server::admit(uint64_t id, SJob &j) {
   const uint64_t states = model ? (7ull << tb) : 0;
   const uint64_t in_off = 0, out_off = up256(q.in_len), st_off = out_off + up256(out_bytes);
   const uint64_t hneed = st_off + up256(states);
-  const uint64_t dneed = q.op == CUCM_OP_COMPRESS ? C * cap : 0;
+  // device staging: compress payloads, or decompress chunks' decoded bytes
+  const uint64_t dneed = q.op == CUCM_OP_COMPRESS ? C * cap : C * S;
   if (hneed > harena.cap || dneed > darena.cap) return CUCM_E_UNSUPPORTED;
   uint64_t hoff = 0, doff = 0;
   if (!harena.alloc(hneed, hoff)) return CUCM_E_NOTREADY;
@@ -490,7 +499,7 @@ int Server::admit(uint64_t id, SJob &j) {
     } else {
       t.src = (uint64_t)(uintptr_t)harena_mem.dev<uint8_t>(hb + in_off + pays[c].first);
       t.aux = pays[c].second;
-      t.dst = (uint64_t)(uintptr_t)harena_mem.dev<uint8_t>(out + P + c * S);
+      t.dst = (uint64_t)(uintptr_t)((uint8_t *)staging.p + doff + c * S);
       t.flags = (uint8_t)(kTaskDecode | mflag);
     }
     ring[(first + c) & (kRing - 1)] = t;
@@ -499,7 +508,8 @@ int Server::admit(uint64_t id, SJob &j) {
   sj.out_base = (uint64_t)(uintptr_t)harena_mem.dev<uint8_t>(out);
   sj.first = first;
   sj.ntasks = (uint32_t)C;
-  sj.header_len = (uint32_t)head_bytes.size();
+  // compress: the header's length; decompress: where chunk 0's bytes go
+  sj.header_len = (uint32_t)(q.op == CUCM_OP_COMPRESS ? head_bytes.size() : P);
   sj.batch = batch;
   sj.id = j.gen;
   publish(j, first, (uint32_t)C);
@@ -552,10 +562,14 @@ int Server::launch_kernel() {
   s.jobcount = (uint32_t *)jobcount.p;
   s.done = done_mem.dev<uint32_t>(done_w);
   s.lens = lens_mem.dev<uint64_t>(lens);
+  s.mirror = (uint32_t *)((uint8_t *)head.p + 64);  // a line of its own, past the claim head
   s.ring_mask = kRing - 1;
   s.job_mask = J - 1;
   s.idle_ticks = kIdleTicks;
-  if (launch_serve(s, shape.blocks, shape.wpb, ks) != hipSuccess) {
+  s.poll_ticks = kPollTicks;
+  // the mirror's stop word still holds the last launch's stop
+  if (hipMemsetAsync(s.mirror + 1, 0, 4, ks) != hipSuccess ||
+      launch_serve(s, shape.blocks, shape.wpb, ks) != hipSuccess) {
     (void)hipGetLastError();
     return CUCM_E_DEVICE;
   }
@@ -713,10 +727,12 @@ void serve_shutdown(cucm_ctx *c) {
 
 extern "C" {
 
-int cucm_serve_start(cucm_ctx *ctx, uint32_t lanes, uint32_t max_inflight) {
-  if (!ctx || lanes > 16 || max_inflight > 4096) return CUCM_E_PARAM;
+int cucm_serve_start_ex(cucm_ctx *ctx, uint32_t lanes, uint32_t max_inflight, uint32_t flags) {
+  if (!ctx || lanes > 16 || max_inflight > 4096 || (flags & ~CUCM_SERVE_STATIC_LANES))
+    return CUCM_E_PARAM;
   if (ctx->server) return CUCM_E_PARAM;
-  if (lanes == 0) lanes = 1;
+  const bool static_lanes = (flags & CUCM_SERVE_STATIC_LANES) != 0;
+  if (lanes == 0) lanes = static_lanes ? 3 : 1;
   uint32_t J = 256;
   if (max_inflight) {
     J = 1;
@@ -725,16 +741,17 @@ int cucm_serve_start(cucm_ctx *ctx, uint32_t lanes, uint32_t max_inflight) {
   auto *s = new (std::nothrow) Server();
   if (!s) return CUCM_E_NOMEM;
   s->parent = ctx;
-  const int rc = s->init_engine(J);
+  const int rc = static_lanes ? CUCM_E_UNSUPPORTED : s->init_engine(J);
   if (rc) {
-    // without the serving kernel every request runs on the fallback lanes
+    // without the serving kernel every request runs on the lanes
     s->engine_ok = false;
     if (rc == CUCM_E_DEVICE || rc == CUCM_E_NOMEM) {
       delete s;
       return rc;
     }
   }
-  s->lane_budget = ctx->budget / (4 * lanes);
+  // the lanes share the budget; beside the serving kernel they get a quarter
+  s->lane_budget = ctx->budget / ((static_lanes ? 1 : 4) * lanes);
   for (uint32_t i = 0; i < lanes; ++i) {
     auto *c = new (std::nothrow) cucm_ctx();
     if (!c) {
@@ -754,6 +771,10 @@ int cucm_serve_start(cucm_ctx *ctx, uint32_t lanes, uint32_t max_inflight) {
   s->sched = std::thread([s] { s->sched_loop(); });
   ctx->server = s;
   return CUCM_OK;
+}
+
+int cucm_serve_start(cucm_ctx *ctx, uint32_t lanes, uint32_t max_inflight) {
+  return cucm_serve_start_ex(ctx, lanes, max_inflight, 0);
 }
 
 int cucm_serve_stop(cucm_ctx *ctx) {

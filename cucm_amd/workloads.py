@@ -66,7 +66,8 @@ def batch_workload(args):
     autotuned with --autotune).  Beside it: the plain path (one launch per
     file, as the reference issues one launch sequence per batch) on a seeded
     sample of the same files, the CPU baseline (the oracle, files as tasks,
-    all host cores and one), and the roofline of the dominant kernel."""
+    all host cores and one), the roofline of the dominant kernel, and the
+    serving mix's latencies (serve_measure)."""
     from .autotune import apply_launch, autotune_batch, load_settings, save_settings
     count = getattr(args, "files", 10000) or 10000
     t0 = time.time()
@@ -179,6 +180,15 @@ def batch_workload(args):
     if not getattr(args, "no_cpu", False):
         res["cpu_baseline"], res["oracle_sample_identical"] = batch_cpu_baseline(
             arena, offs, sizes, d_c, coffs, cl, s_max, tb, args)
+    # the serving mix (Poisson arrivals, lognormal sizes): the serving kernel
+    # against static-batch lanes at the same loads
+    nreq = getattr(args, "serve_requests", 240)
+    res["serve"] = None
+    if nreq:
+        for b in (d_in, d_c, d_o):
+            b.free()
+        apply_launch(ctx, {"team": None, "waves_per_block": 0})  # automatic, as for any caller
+        res["serve"] = serve_measure(ctx, nreq, loads=(0.5, 1.0))
     return res
 
 
@@ -314,59 +324,58 @@ def _fifo_latencies(arrivals, service):
     return lat
 
 
-def serve_workload(args):
-    """Poisson arrivals of mixed requests through the serving loop.
-
-    Requests: 50% compress of fresh payloads, 50% decompress of containers
-    made in a warm-up; payload sizes min(round(256 KiB * exp(1.2 Z)), 128 MiB),
-    content from the order-2 Markov source (request i seeded 5000 + i); the
-    small-config model (16 KiB chunks, 2^18 slots, U = 128).  First the
-    saturated rate is measured by submitting every request at once; then
-    arrivals are replayed at lambda = f x saturated MB/s / mean size for each
-    load factor f.  The plain path (one launch per request, one request at a
-    time) is timed per request and its FIFO latencies computed for the same
-    arrival times.  Every output is compared with the same request run alone.
-    """
-    import threading
-    count = getattr(args, "requests", 240) or 240
+def serve_requests(count, seed=11):
+    """The serving mix: `count` requests, 50% compress of fresh payloads and
+    50% decompress of containers made beforehand; payload sizes
+    min(round(256 KiB * exp(1.2 Z)), 128 MiB), content from the order-2
+    Markov source (request i seeded 5000 + i); 16 KiB chunks, 2^18 slots,
+    U = 128.  Returns (requests, params, containers' inputs)."""
     S, tb = 16384, 18
     params = make_params(S, 128, 100, 0, tb)
-    rng = np.random.default_rng(11)
+    rng = np.random.default_rng(seed)
     sizes = datagen.lognormal_sizes(count, 262144, 1.2, 1, 128 * MiB, 13)
     pool_sizes = datagen.lognormal_sizes(count // 2 + 1, 262144, 1.2, 1, 128 * MiB, 17)
     arena, offs = datagen.markov2_files(sizes, seed0=5000)
     parena, poffs = datagen.markov2_files(pool_sizes, seed0=9000)
     fresh = [arena[int(o):int(o) + int(n)] for o, n in zip(offs, sizes)]
     pool = [parena[int(o):int(o) + int(n)] for o, n in zip(poffs, pool_sizes)]
-    ctx = Context()
-    # warm-up: containers that the decompress requests replay
-    blobs = ctx.compress_many(pool, params)
     ops = rng.random(count) < 0.5
+    picks = rng.integers(len(pool), size=count)
+    return fresh, pool, ops, picks, params
+
+
+def serve_measure(ctx, count=240, loads=(0.25, 0.5, 0.75, 0.9, 1.0, 1.2), lanes=0):
+    """Poisson arrivals of the serving mix through both serving designs: the
+    serving kernel (requests' tasks admitted into one running launch) and
+    static batches on lanes (each lane runs the oldest pending requests of
+    one kind as one batch to the end).  First each design's saturated rate is
+    measured by submitting every request at once; then the same arrival
+    times, at lambda = f x (the lanes' saturated MB/s) / mean size for each
+    load factor f, are replayed through both.  The plain path (one launch per
+    request, one request at a time) is timed per request and its FIFO
+    latencies computed for the same arrival times.  Every output is compared
+    with the same request run alone."""
+    import threading
+    fresh, pool, ops, picks, params = serve_requests(count)
+    blobs = ctx.compress_many(pool, params)
     reqs = []
     for i in range(count):
         if ops[i]:
             reqs.append(("c", fresh[i], len(fresh[i])))
         else:
-            k = int(rng.integers(len(pool)))
-            reqs.append(("d", (blobs[k], k), len(pool[k])))
+            k = int(picks[i])
+            reqs.append(("d", blobs[k], len(pool[k])))
     mean_bytes = float(np.mean([r[2] for r in reqs]))
     total_bytes = float(sum(r[2] for r in reqs))
     # results of each request run alone, and the plain path's service times
     alone, service = [], []
     for kind, x, n in reqs:
         t0 = time.perf_counter()
-        if kind == "c":
-            out = ctx.compress(x, params)
-        else:
-            out = ctx.decompress(x[0])
+        out = ctx.compress(x, params) if kind == "c" else ctx.decompress(x)
         service.append(time.perf_counter() - t0)
         alone.append(out)
     plain_MBps = total_bytes / 1e6 / sum(service)
     log("serve: %d requests, %.1f MB, plain path %.1f MB/s" % (count, total_bytes / 1e6, plain_MBps))
-
-    # one long-running server for every load level; its lanes' buffers are
-    # grown by a first saturated pass that is not reported
-    ctx.serve_start(args.lanes, 0)
 
     def run(arrivals):
         launches0 = ctx.serve_launches()
@@ -387,7 +396,7 @@ def serve_workload(args):
                 d = t0 + arrivals[i] - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-            ids[i] = ctx.submit_compress(x, params) if kind == "c" else ctx.submit_decompress(x[0])
+            ids[i] = ctx.submit_compress(x, params) if kind == "c" else ctx.submit_decompress(x)
         th.join()
         makespan = time.perf_counter() - t0
         launches = ctx.serve_launches() - launches0
@@ -397,22 +406,49 @@ def serve_workload(args):
                 "launches": launches, "mean_batch": float(np.mean([r[1]["batch_jobs"] for r in results])),
                 "identical_to_alone": bool(same), "makespan_s": makespan}
 
-    run(None)
-    sat = run(None)
-    log("serve saturated: %.1f MB/s, %d launches" % (sat["MBps"], sat["launches"]))
+    designs = (("kernel", False), ("lanes", True))
+    sat = {}
+    for name, static in designs:
+        ctx.serve_start(lanes, 0, static_lanes=static)
+        run(None)  # grows the lanes' buffers; not reported
+        sat[name] = run(None)
+        ctx.serve_stop()
+        log("serve %s saturated: %.1f MB/s p50 %.1f ms p99 %.1f ms, %d launches"
+            % (name, sat[name]["MBps"], sat[name]["p50_ms"], sat[name]["p99_ms"], sat[name]["launches"]))
+    base = sat["lanes"]["MBps"]
     sweeps = []
-    for f in [0.25, 0.5, 0.75, 0.9, 1.0, 1.2]:
-        lam = f * sat["MBps"] * 1e6 / mean_bytes
+    for f in loads:
+        lam = f * base * 1e6 / mean_bytes
         arr = datagen.poisson_arrivals(count, lam, 19)
-        r = run(arr)
         plain = [1000.0 * x for x in _fifo_latencies(arr, service)]
-        r.update({"load": f, "lambda_per_s": lam, "offered_MBps": lam * mean_bytes / 1e6,
-                  "plain_p50_ms": _pct(plain, 50), "plain_p99_ms": _pct(plain, 99)})
-        log("serve load %.2f: %.1f MB/s p50 %.1f ms p99 %.1f ms (plain p50 %.1f p99 %.1f)"
-            % (f, r["MBps"], r["p50_ms"], r["p99_ms"], r["plain_p50_ms"], r["plain_p99_ms"]))
-        sweeps.append(r)
-    ctx.serve_stop()
-    ok = sat["identical_to_alone"] and all(r["identical_to_alone"] for r in sweeps)
+        row = {"load": f, "lambda_per_s": lam, "offered_MBps": lam * mean_bytes / 1e6,
+               "plain_p50_ms": _pct(plain, 50), "plain_p99_ms": _pct(plain, 99)}
+        sweeps.append(row)
+    for name, static in designs:
+        ctx.serve_start(lanes, 0, static_lanes=static)
+        run(None)
+        for row in sweeps:
+            arr = datagen.poisson_arrivals(count, row["lambda_per_s"], 19)
+            row[name] = run(arr)
+            log("serve %s load %.2f: %.1f MB/s p50 %.1f ms p99 %.1f ms"
+                % (name, row["load"], row[name]["MBps"], row[name]["p50_ms"], row[name]["p99_ms"]))
+        ctx.serve_stop()
+    ok = all(sat[n]["identical_to_alone"] for n, _ in designs) and \
+        all(r[n]["identical_to_alone"] for r in sweeps for n, _ in designs)
+    return {"requests": count, "bytes": int(total_bytes), "mean_request_bytes": mean_bytes,
+            "saturated": sat, "plain_MBps": plain_MBps,
+            "plain_p50_ms": _pct([1000 * x for x in service], 50),
+            "sweep": sweeps, "identical_to_alone": bool(ok),
+            "load_basis": "load f = arrivals at f x the lanes' saturated MB/s"}
+
+
+def serve_workload(args):
+    """The serving mix (serve_measure) as its own bench line: value is the
+    serving kernel's saturated MB/s."""
+    count = getattr(args, "requests", 240) or 240
+    ctx = Context()
+    r = serve_measure(ctx, count, lanes=args.lanes)
+    sat = r["saturated"]["kernel"]
     return {
         "metric": "compress/decompress MB/s on 1 GPU at identical ratio; batched vs plain; p99",
         "value": sat["MBps"], "unit": "MB/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -421,9 +457,8 @@ def serve_workload(args):
         "data": "synthetic (seeded generator, host buffers, PCIe included)",
         "config": {"workload": "serve: %d Poisson requests, 50%% compress / 50%% decompress, "
                                "lognormal sizes (median 256 KiB, sigma 1.2), 16 KiB chunks, "
-                               "2^18 slots, U=128" % count, "lanes": args.lanes or 3},
-        "saturated": sat, "plain_MBps": plain_MBps, "plain_p50_ms": _pct([1000 * s for s in service], 50),
-        "sweep": sweeps, "identical_to_alone": bool(ok),
+                               "2^18 slots, U=128" % count},
+        "serve": r, "identical_to_alone": r["identical_to_alone"],
     }
 
 

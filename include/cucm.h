@@ -262,14 +262,26 @@ typedef struct cucm_job {
 } cucm_job;
 typedef struct cucm_result {
   int32_t status;          /* CUCM_OK or CUCM_E_* */
-  int32_t batch_jobs;      /* requests that shared this request's launch */
+  int32_t batch_jobs;      /* requests on the GPU with this one when it was admitted
+                              (serving kernel), or in its batch (lanes) */
   uint64_t out_len;
-  double queue_ms;         /* submit -> taken by a lane */
+  double queue_ms;         /* submit -> admitted (tasks published, or taken by a lane) */
   double latency_ms;       /* submit -> result in the caller's buffer */
 } cucm_result;
-/* Starts the serving loop on ctx: lanes (0 = 3, at most 16) concurrent
- * launches, max_inflight (0 = 4096) requests taken by lanes at once. */
+/* Starts the serving loop on ctx.  One persistent launch of the serving
+ * kernel takes the chunk tasks of every admitted request, compress and
+ * decompress mixed, as its waves free up; a request is admitted as soon as
+ * one of max_inflight (0 = 256, rounded up to a power of 2, at most 4096) job
+ * slots and its memory are free.  Requests the kernel does not take (frozen
+ * model, U above 128, tables above 2^20 slots, or larger than the request
+ * arena) run as static batches on `lanes` (0 = 1, at most 16) concurrent
+ * lanes.  No counterpart in the reference, which has no serving loop. */
 int cucm_serve_start(cucm_ctx *ctx, uint32_t lanes, uint32_t max_inflight);
+/* cucm_serve_start with flags: CUCM_SERVE_STATIC_LANES runs every request as
+ * static batches on the lanes (lanes 0 = 3), without the serving kernel --
+ * the loop's earlier design, kept for comparison. */
+#define CUCM_SERVE_STATIC_LANES 1u
+int cucm_serve_start_ex(cucm_ctx *ctx, uint32_t lanes, uint32_t max_inflight, uint32_t flags);
 /* Waits until every submitted request has completed, then stops the lanes.
  * Must not run concurrently with cucm_submit/cucm_poll on the same ctx. */
 int cucm_serve_stop(cucm_ctx *ctx);

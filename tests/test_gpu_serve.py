@@ -1,7 +1,9 @@
 """Serving loop (cucm_serve_start / cucm_submit / cucm_poll): requests of
 mixed kinds, sizes and parameters (frozen-model ones among them) submitted
-together are batched into shared launches on concurrent lanes, and every result equals the same request run
-alone -- bit-exact, since the codec's output depends only on the request."""
+together share the serving kernel's running launch (or, with static lanes,
+shared batches), requests that arrive mid-launch are admitted into it, and
+every result equals the same request run alone -- bit-exact, since the
+codec's output depends only on the request."""
 import numpy as np
 import pytest
 
@@ -32,14 +34,15 @@ def _requests():
     return reqs
 
 
-def test_serve_results_identical_alone_and_in_mix(ctx):
+@pytest.mark.parametrize("static_lanes", [False, True], ids=["kernel", "static-lanes"])
+def test_serve_results_identical_alone_and_in_mix(ctx, static_lanes):
     reqs = _requests()
     alone_c = [ctx.compress(d, p) for d, p in reqs]
     for (d, _), b in zip(reqs, alone_c):
         assert ctx.decompress(b) == d
     sc = cucm_amd.Context()
     try:
-        sc.serve_start(lanes=3)
+        sc.serve_start(lanes=3, static_lanes=static_lanes)
         ids = []
         for i, (d, p) in enumerate(reqs):
             ids.append(("c", i, sc.submit_compress(d, p)))
