@@ -240,8 +240,7 @@ CUCM_DEV void big_encode_task(const CodecArgs &a, const Task &tk, uint32_t t, co
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
         const uint32_t hk = bi >= (uint64_t)k ? (uint32_t)src[bi - k] : 0u;
-        H = (H ^ hk) * kFnvPrime;
-        sl[k] = H & mask;
+        sl[k] = fnv_pos(H, hk, mask);
       }
       g[0] = M.st0[h0];
 #pragma unroll
@@ -322,11 +321,14 @@ CUCM_DEV void big_decode_task(const CodecArgs &a, const Task &tk, uint32_t t, co
       }
       const uint32_t j = (uint32_t)((base + r) & 7u);
       const uint32_t h0 = (1u << j) | partial;
-      uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
+      uint32_t H = (kFnvBasis ^ h0) * kFnvPrime, pos = 0;
 #pragma unroll
-      for (int k = 1; k < 8; ++k)
-        H = (uint32_t)k <= kk ? (H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u)) * kFnvPrime : H;
-      const uint32_t sl = kk ? (H & mask) : h0;
+      for (int k = 1; k < 8; ++k) {
+        const uint32_t x = H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u);
+        pos = (uint32_t)k == kk ? x : pos;
+        H = (uint32_t)k < kk ? x * kFnvPrime : H;
+      }
+      const uint32_t sl = kk ? (pos & mask) : h0;
       // every lane loads (lane 0 a byte of table 0, in bounds) and selects
       const uint32_t gh = slot[((kk ? kk - 1 : 0u) << tb) | sl];
       const uint32_t g = kk ? gh : (uint32_t)M.st0[h0];

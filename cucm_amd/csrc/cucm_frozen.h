@@ -124,8 +124,7 @@ CUCM_DEV void frozen_block_issue(const FrozenMem &F, const GLB uint8_t *states, 
       uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
-        H = (H ^ h[k]) * kFnvPrime;
-        g[j][k] = states[((uint32_t)(k - 1) << tb) | (H & mask)];
+        g[j][k] = states[((uint32_t)(k - 1) << tb) | fnv_pos(H, h[k], mask)];
       }
     }
   } else {
@@ -225,10 +224,10 @@ CUCM_DEV void frozen_paths_issue(const FrozenMem &F, const GLB uint8_t *states, 
       uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
-        H = (H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u)) * kFnvPrime;
         // lanes without a path read the table's first byte; the offset
-        // (k - 1) * 2^tb + slot < 7 * 2^25 fits 32 bits
-        g[m][k] = states[((uint32_t)(k - 1) << tb) | (H & lmask)];
+        // (k - 1) * 2^tb + position < 7 * 2^25 fits 32 bits
+        g[m][k] = states[((uint32_t)(k - 1) << tb) |
+                         fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), lmask)];
       }
     } else {
 #pragma unroll

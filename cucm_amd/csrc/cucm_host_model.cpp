@@ -76,8 +76,12 @@ void eval(const HostModel &m, const DevTables &T, const uint8_t *buf, uint64_t p
       ctx = (1u << j) | (j ? (uint32_t)buf[byte] >> (8 - j) : 0u);
     else
       ctx = pos >= 8ull * k ? buf[byte - k] : 0u;
-    h = (h ^ ctx) * kFnvPrime;
-    e.slot[k] = h & m.mask;
+    if (k == 0) {
+      h = (h ^ ctx) * kFnvPrime;
+      e.slot[0] = h & m.mask;
+    } else {
+      e.slot[k] = fnv_pos(h, ctx, m.mask);  // the device tables' layout
+    }
   }
   uint32_t active = 0;
   for (int k = 0; k < kOrders; ++k) {

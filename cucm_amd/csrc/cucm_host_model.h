@@ -12,7 +12,9 @@ void build_tables(DevTables *t);
 const DevTables &host_tables();
 
 // Full model on the host: 8 order tables of 2^tb one-byte states, statep and
-// mixer weights (StateMap/Mixer, cucm_compress_v2.cu:558-679).
+// mixer weights (StateMap/Mixer, cucm_compress_v2.cu:558-679).  Order 0 is
+// indexed by slot; orders 1..7 are laid out as on the device (fnv_pos), so
+// they upload as one copy.
 struct HostModel {
   explicit HostModel(int tb);
   int tb;

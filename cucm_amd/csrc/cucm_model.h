@@ -16,6 +16,22 @@ constexpr uint32_t kTeamWaves = 4; // waves of a team workgroup (one chunk per w
 constexpr uint32_t kFnvBasis = 2166136261u;
 constexpr uint32_t kFnvPrime = 16777619u;
 
+// Where order k's state (k >= 1) sits in its table.  The reference indexes
+// order k by slot = H_k & mask with H_k = (H_{k-1} ^ b_k) * prime
+// (cucm_compress_v2.cu:727-743).  Here the state of slot s is stored at
+// position s * prime^-1 mod 2^tb, which is (H_{k-1} ^ b_k) & mask: prime is
+// odd, so the map is a bijection of the table (two contexts share a state
+// exactly when they share a slot, and every coded bit is unchanged), and the
+// 256 contexts that differ only in their farthest byte fall into one aligned
+// 256-byte block -- order 1's states of a chunk lie in at most 255 such
+// blocks, so a far larger share of the state traffic hits in cache.
+// Advances H from H_{k-1} to H_k and returns order k's position.
+constexpr uint32_t fnv_pos(uint32_t &H, uint32_t b, uint32_t mask) {
+  const uint32_t x = H ^ b;
+  H = x * kFnvPrime;
+  return x & mask;
+}
+
 // Bit-history transition nex[s] = next0 | next1 << 8.  Every entry must
 // equal the reference's state_table (cucm_compress_v2.cu:35-285): the coded
 // stream depends on all of them.

@@ -241,8 +241,7 @@ CUCM_DEV void paths_issue_gathers(const GLB uint8_t *slot, uint32_t tb, uint32_t
     sl[m][0] = h0;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
-      H = (H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u)) * kFnvPrime;
-      sl[m][k] = H & lmask;
+      sl[m][k] = fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), lmask);
     }
 #pragma unroll
     for (int k = 1; k < 8; ++k) g[m][k] = slot[((uint32_t)(k - 1) << tb) | sl[m][k]];
@@ -335,8 +334,7 @@ CUCM_DEV void team_candidates(LDS uint32_t *cand, uint32_t mask, uint32_t j0, ui
     sl[0] = h0;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
-      H = (H ^ (uint32_t)((hn >> (8 * (k - 1))) & 255u)) * kFnvPrime;
-      sl[k] = H & mask;
+      sl[k] = fnv_pos(H, (uint32_t)((hn >> (8 * (k - 1))) & 255u), mask);
     }
     LDS u32x4 *d = (LDS u32x4 *)(cand + c * 8u);
     d[0] = u32x4{sl[0], sl[1], sl[2], sl[3]};

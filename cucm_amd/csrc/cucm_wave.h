@@ -638,8 +638,7 @@ CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
     sl[j][0] = h0;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
-      H = (H ^ (uint32_t)M.ub[8 + bi - k]) * kFnvPrime;
-      sl[j][k] = H & mask;
+      sl[j][k] = fnv_pos(H, (uint32_t)M.ub[8 + bi - k], mask);
     }
     g[j][0] = M.st0[h0];
 #pragma unroll
@@ -901,8 +900,7 @@ CUCM_DEV void paths_issue(const WaveMem &M, const GLB uint8_t *slot, uint32_t tb
     sl[0] = h0;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
-      H = (H ^ (uint32_t)((hist >> (8 * (k - 1))) & 255u)) * kFnvPrime;
-      sl[k] = H & lmask;
+      sl[k] = fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), lmask);
     }
     g[m][0] = M.st0[h0];
     // unconditional loads (a predicated load would be waited for at the

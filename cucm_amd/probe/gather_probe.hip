@@ -65,7 +65,8 @@ __global__ void copy_kernel(const uint4 *src, uint4 *dst, uint64_t n16) {
 // model slot (7 order tables of 2^tb bytes), then for every round of D bits
 // lane q < 2^D - 1 hashes speculative path q of the round exactly as the
 // decoder does (partial byte plus the path's bits as h0, the FNV chain over
-// the 7 previous bytes of the chunk) and gathers its 7 states; the next
+// the 7 previous bytes of the chunk, each state at the codec's table
+// position (H_{k-1} ^ b_k) & mask) and gathers its 7 states; the next
 // round's addresses depend on this round's loads (so rounds are dependent
 // round trips, as in the decoder), and after every unit of U bits the 7
 // slots of each of the unit's bits are written back.  The taken path of a
@@ -127,11 +128,14 @@ __global__ void __launch_bounds__(256) replay_decode_kernel(const ReplayChunk *c
           const uint32_t plen = 31u - (uint32_t)__clz((int)(q + 1u));
           const uint32_t pbits = q + 1u - (1u << plen);
           const uint32_t h0 = ((1u << (j0 + plen)) | (partial << plen) | pbits) & 255u;
-          uint32_t H = (2166136261u ^ h0) * 16777619u;
+          uint32_t H = (2166136261u ^ h0) * 16777619u, pos = 0;
 #pragma unroll
-          for (int k = 1; k < 8; ++k)
-            H = (uint32_t)k <= kk ? (H ^ (uint32_t)ub[8 + bi - k]) * 16777619u : H;
-          sum = slot[((kk - 1) << tb) | ((H + dep) & mask)];
+          for (int k = 1; k < 8; ++k) {
+            const uint32_t x = H ^ (uint32_t)ub[8 + bi - k];
+            pos = (uint32_t)k == kk ? x : pos;
+            H = (uint32_t)k < kk ? x * 16777619u : H;
+          }
+          sum = slot[((kk - 1) << tb) | ((pos + dep) & mask)];
           if (P > 9) {
             // paths 9.. in a second instruction
             const uint32_t lq2 = 9u + lane / 7u;
@@ -140,11 +144,14 @@ __global__ void __launch_bounds__(256) replay_decode_kernel(const ReplayChunk *c
             const uint32_t plen2 = 31u - (uint32_t)__clz((int)(q2 + 1u));
             const uint32_t pbits2 = q2 + 1u - (1u << plen2);
             const uint32_t h02 = ((1u << (j0 + plen2)) | (partial << plen2) | pbits2) & 255u;
-            uint32_t H2 = (2166136261u ^ h02) * 16777619u;
+            uint32_t H2 = (2166136261u ^ h02) * 16777619u, pos2 = 0;
 #pragma unroll
-            for (int k = 1; k < 8; ++k)
-              H2 = (uint32_t)k <= kk ? (H2 ^ (uint32_t)ub[8 + bi - k]) * 16777619u : H2;
-            sum += slot[((kk - 1) << tb) | ((H2 + dep) & mask)];
+            for (int k = 1; k < 8; ++k) {
+              const uint32_t x = H2 ^ (uint32_t)ub[8 + bi - k];
+              pos2 = (uint32_t)k == kk ? x : pos2;
+              H2 = (uint32_t)k < kk ? x * 16777619u : H2;
+            }
+            sum += slot[((kk - 1) << tb) | ((pos2 + dep) & mask)];
           }
         } else {
           const uint32_t q = lane < P ? lane : 0u;
@@ -154,8 +161,9 @@ __global__ void __launch_bounds__(256) replay_decode_kernel(const ReplayChunk *c
           uint32_t H = (2166136261u ^ h0) * 16777619u;
 #pragma unroll
           for (int k = 1; k < 8; ++k) {
-            H = (H ^ (uint32_t)ub[8 + bi - k]) * 16777619u;
-            sum += slot[((uint32_t)(k - 1) << tb) | ((H + dep) & mask)];
+            const uint32_t x = H ^ (uint32_t)ub[8 + bi - k];
+            H = x * 16777619u;
+            sum += slot[((uint32_t)(k - 1) << tb) | ((x + dep) & mask)];
           }
         }
         // the next round waits for this one's loads (sum < 2^31 always)
@@ -167,8 +175,9 @@ __global__ void __launch_bounds__(256) replay_decode_kernel(const ReplayChunk *c
           uint32_t H = (2166136261u ^ h0) * 16777619u;
 #pragma unroll
           for (int k = 1; k < 8; ++k) {
-            H = (H ^ (uint32_t)ub[8 + bi - k]) * 16777619u;
-            if (lane == 0) slot[((uint32_t)(k - 1) << tb) | ((H + dep) & mask)] = (uint8_t)(rb + dep);
+            const uint32_t x = H ^ (uint32_t)ub[8 + bi - k];
+            H = x * 16777619u;
+            if (lane == 0) slot[((uint32_t)(k - 1) << tb) | ((x + dep) & mask)] = (uint8_t)(rb + dep);
           }
         }
       }
@@ -182,8 +191,9 @@ __global__ void __launch_bounds__(256) replay_decode_kernel(const ReplayChunk *c
           uint32_t H = (2166136261u ^ h0) * 16777619u;
 #pragma unroll
           for (int k = 1; k < 8; ++k) {
-            H = (H ^ (uint32_t)ub[8 + bi - k]) * 16777619u;
-            s2 += slot[((uint32_t)(k - 1) << tb) | (H & mask)];
+            const uint32_t x = H ^ (uint32_t)ub[8 + bi - k];
+            H = x * 16777619u;
+            s2 += slot[((uint32_t)(k - 1) << tb) | (x & mask)];
           }
         }
         dep = __builtin_amdgcn_readfirstlane(s2) == 0x7fffffffu ? 1u : dep;
@@ -196,8 +206,9 @@ __global__ void __launch_bounds__(256) replay_decode_kernel(const ReplayChunk *c
         uint32_t H = (2166136261u ^ h0) * 16777619u;
 #pragma unroll
         for (int k = 1; k < 8; ++k) {
-          H = (H ^ (uint32_t)ub[8 + bi - k]) * 16777619u;
-          slot[((uint32_t)(k - 1) << tb) | (H & mask)] = (uint8_t)(r + dep);
+          const uint32_t x = H ^ (uint32_t)ub[8 + bi - k];
+          H = x * 16777619u;
+          slot[((uint32_t)(k - 1) << tb) | (x & mask)] = (uint8_t)(r + dep);
         }
       }
       __builtin_amdgcn_wave_barrier();
