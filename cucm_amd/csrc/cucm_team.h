@@ -73,7 +73,7 @@ struct TeamLayout {
 CUCM_HDI TeamLayout team_layout(uint32_t Umax, uint32_t D) {
   const uint32_t P = (1u << D) - 1u;
   TeamLayout t;
-  t.w = wave_layout(Umax, P, true, team_round_buffers(D));
+  t.w = wave_layout(Umax, P, true, team_round_buffers(D), true);
   uint32_t off = t.w.total;
   t.stpn = off;
   off = align16(off + 8 * 256 * 4);
@@ -101,7 +101,7 @@ CUCM_HD uint32_t team_lds_bytes(uint32_t Umax, uint32_t D) { return team_layout(
 CUCM_DEV TeamMem carve_team(LDS uint8_t *base, uint32_t Umax, uint32_t D, uint32_t wid) {
   const TeamLayout l = team_layout(Umax, D);
   TeamMem x;
-  x.M = carve(base, Umax, (1u << D) - 1u, true, team_round_buffers(D));
+  x.M = carve(base, Umax, (1u << D) - 1u, true, team_round_buffers(D), true);
   x.M.dummy = (LDS uint32_t *)(base + l.dummy) + 64 * wid;
   x.stpn = (LDS uint32_t *)(base + l.stpn);
   x.wd = (LDS int32_t *)(base + l.wd);
@@ -807,7 +807,7 @@ struct TeamEncLayout {
 
 CUCM_HDI TeamEncLayout team_enc_layout(uint32_t Umax) {
   TeamEncLayout t;
-  t.w = wave_layout(Umax, 0, false);
+  t.w = wave_layout(Umax, 0, false, 2, true);
   uint32_t off = t.w.total;
   t.ubw = off;
   off = align16(off + Umax / 8);
@@ -828,7 +828,7 @@ CUCM_HD uint32_t team_enc_lds_bytes(uint32_t Umax) { return team_enc_layout(Umax
 CUCM_DEV TeamMem carve_team_enc(LDS uint8_t *base, uint32_t Umax, uint32_t wid) {
   const TeamEncLayout l = team_enc_layout(Umax);
   TeamMem x;
-  x.M = carve(base, Umax, 0, false);
+  x.M = carve(base, Umax, 0, false, 2, true);
   x.M.dummy = (LDS uint32_t *)(base + l.dummy) + 64 * wid;
   x.stpn = x.M.stp;
   x.wd = nullptr;

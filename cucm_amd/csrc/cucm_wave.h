@@ -183,7 +183,9 @@ struct WaveMem {
   LDS int32_t *w;       // [80][8] mixer weights
   LDS uint32_t *tkey;   // [7][TS] distinct slots of the unit, orders 1..7
   LDS uint32_t *pslot;  // [2][P][8] decoder: slot (order 0: h0) per path, two rounds
-  LDS uint16_t *tval;   // [7][TS] running state of each distinct slot
+  LDS uint16_t *tval;   // teams: [7][TS] running state of each distinct slot (kNoState: untouched)
+  LDS uint8_t *tv8;     // one wave per chunk: [7][TS] running state of each distinct slot,
+                        // set to the unit-start state by every bit that records the slot
   LDS uint16_t *ridx;   // [8][Umax] table index (order 0: h0) per unit bit
   LDS uint16_t *rprm;   // [Umax] pr << 1 | y per unit bit (persists: tail padding)
   LDS uint8_t *st0;     // [256] order-0 states by h0
@@ -195,12 +197,15 @@ struct WaveMem {
 };
 
 // Byte offsets of the per-wave LDS arrays (16-byte aligned each); the
-// decoder keeps nbuf rounds of path slots and states.
+// decoder keeps nbuf rounds of path slots and states.  One wave per chunk
+// keeps the distinct slots' running states as bytes (tv8), teams as 16-bit
+// words with an untouched marker (tval, wide).
 struct WaveLayout {
   uint32_t stp, w, tkey, pslot, tval, ridx, rprm, st0, rg, rcxt, pg, ub, dummy, total;
 };
 
-CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec, uint32_t nbuf = 2) {
+CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec, uint32_t nbuf = 2,
+                                bool wide = false) {
   const uint32_t TS = table_size(Umax);
   const uint32_t Pd = dec ? P : 0u;
   WaveLayout l;
@@ -212,7 +217,7 @@ CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec, uint32_t nb
   CUCM_TAKE(w, kMixCtx * kOrders * 4)
   CUCM_TAKE(tkey, 7 * TS * 4)
   CUCM_TAKE(pslot, nbuf * Pd * 8 * 4)
-  CUCM_TAKE(tval, 7 * TS * 2)
+  CUCM_TAKE(tval, 7 * TS * (wide ? 2 : 1))
   CUCM_TAKE(ridx, 8 * Umax * 2)
   CUCM_TAKE(rprm, Umax * 2)
   CUCM_TAKE(st0, 256)
@@ -231,14 +236,15 @@ CUCM_HD uint32_t wave_lds_bytes(uint32_t Umax, uint32_t P, bool dec) {
 }
 
 CUCM_DEV WaveMem carve(LDS uint8_t *base, uint32_t Umax, uint32_t P, bool dec,
-                       uint32_t nbuf = 2) {
-  const WaveLayout l = wave_layout(Umax, P, dec, nbuf);
+                       uint32_t nbuf = 2, bool wide = false) {
+  const WaveLayout l = wave_layout(Umax, P, dec, nbuf, wide);
   WaveMem m;
   m.stp = (LDS uint32_t *)(base + l.stp);
   m.w = (LDS int32_t *)(base + l.w);
   m.tkey = (LDS uint32_t *)(base + l.tkey);
   m.pslot = (LDS uint32_t *)(base + l.pslot);
-  m.tval = (LDS uint16_t *)(base + l.tval);
+  m.tval = wide ? (LDS uint16_t *)(base + l.tval) : nullptr;
+  m.tv8 = wide ? nullptr : base + l.tval;
   m.ridx = (LDS uint16_t *)(base + l.ridx);
   m.rprm = (LDS uint16_t *)(base + l.rprm);
   m.st0 = base + l.st0;
@@ -353,10 +359,7 @@ CUCM_DEV void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M, G
     for (uint32_t i = lane; i < 256; i += 64) M.st0[i] = 0;
   }
   const uint32_t TS = table_size(Umax);
-  for (uint32_t i = lane; i < 7 * TS; i += 64) {
-    M.tkey[i] = kEmpty;
-    M.tval[i] = kNoState;
-  }
+  for (uint32_t i = lane; i < 7 * TS; i += 64) M.tkey[i] = kEmpty;
   for (uint32_t i = lane; i < Umax; i += 64) M.rprm[i] = 0;
   for (uint32_t i = lane; i < 8 + Umax / 8 + 8; i += 64) M.ub[i] = 0;
   vm_drain();
@@ -431,7 +434,7 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
   // 2. state resolution, one byte of the unit per step
   {
     const uint32_t k = lane >> 3, j = lane & 7u;
-    LDS uint16_t *tv = M.tval + (k ? (k - 1) * TS : 0);
+    LDS uint8_t *tv = M.tv8 + (k ? (k - 1) * TS : 0);
 #pragma unroll
     for (int seg = 0; seg < NJ; ++seg) {
       const uint64_t yseg = ym[seg];
@@ -444,18 +447,13 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
         const uint32_t reps = ((cmask >> b) & 1u) ? 8u : 1u;
         for (uint32_t q = 0; q < reps; ++q) {
           if (reps == 1u || q == j) {
-            uint32_t st;
-            if (k == 0) {
-              st = M.st0[i];
-            } else {
-              const uint32_t v = tv[i];
-              st = v > 255u ? g : v;
-            }
+            // (tv[i] starts the unit as the slot's gathered state)
+            const uint32_t st = k == 0 ? M.st0[i] : tv[i];
             const uint32_t ns = T->nex[2 * st + y];
             if (k == 0)
               M.st0[i] = (uint8_t)ns;
             else
-              tv[i] = (uint16_t)ns;
+              tv[i] = (uint8_t)ns;
             M.rg[k * Umax + r] = (uint8_t)st;
           }
           wave_sync();
@@ -471,7 +469,7 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
     GLB uint8_t *tab = slot + ((uint64_t)(k - 1) << tb);
     for (uint32_t r = lane; r < nvalid; r += 64) {
       const uint32_t i = M.ridx[k * Umax + r];
-      tab[M.tkey[(k - 1) * TS + i]] = (uint8_t)M.tval[(k - 1) * TS + i];
+      tab[M.tkey[(k - 1) * TS + i]] = M.tv8[(k - 1) * TS + i];
     }
   }
   wave_sync();
@@ -480,7 +478,6 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
     for (uint32_t r = lane; r < nvalid; r += 64) {
       const uint32_t i = M.ridx[k * Umax + r];
       M.tkey[(k - 1) * TS + i] = kEmpty;
-      M.tval[(k - 1) * TS + i] = kNoState;
     }
   vm_drain();
   wave_sync();
@@ -692,6 +689,7 @@ CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
                                                uint32_t nvalid, uint32_t Umax, uint32_t lane,
                                                const uint32_t sl[NJ][8], const uint32_t g[NJ][8],
                                                const uint32_t idx[NJ][8]) {
+  const uint32_t TS = table_size(Umax);
   uint32_t pv[NJ][8];
 #pragma unroll
   for (int j = 0; j < NJ; ++j)
@@ -732,6 +730,8 @@ CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
       for (int k = 1; k < 8; ++k) {
         M.ridx[k * Umax + rr] = (uint16_t)idx[j][k];
         M.rg[k * Umax + rr] = (uint8_t)g[j][k];
+        // every bit of a slot gathered the same unit-start state
+        M.tv8[(k - 1) * TS + idx[j][k]] = (uint8_t)g[j][k];
       }
     }
   }
@@ -1074,6 +1074,8 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
           M.rg[k * Umax + r] = (uint8_t)gq;
           M.ridx[k * Umax + r] = (uint16_t)(k == 0 ? sl : h);
         }
+        // the slot's unit-start state (every round of the unit gathered the same)
+        if (ins) M.tv8[(k - 1) * TS + h] = (uint8_t)gq;
         if (lane < (uint32_t)D) {
           const uint64_t pw = lane < 4 ? prmp[0] : prmp[1];
           M.rprm[rb + lane] = (uint16_t)(pw >> (16 * (lane & 3u)));
