@@ -213,21 +213,19 @@ CUCM_DEV void frozen_paths_issue(const FrozenMem &F, const GLB uint8_t *states, 
                                  uint32_t lane, uint32_t g[NM][8]) {
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
-    const uint32_t q = lane + 64u * m;
-    const bool ok = q < P;
+    // lanes without a path repeat path 0 (its lines, in bounds)
+    const uint32_t q = lane + 64u * m < P ? lane + 64u * m : 0u;
     const uint32_t plen = 31u - clz32(q + 1u);
     const uint32_t pbits = q + 1u - (1u << plen);
     const uint32_t h0 = ((1u << (j0 + plen)) | (partial << plen) | pbits) & 255u;
     g[m][0] = F.st0[h0];
     if (states) {
-      const uint32_t lmask = ok ? mask : 0u;
       uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
-        // lanes without a path read the table's first byte; the offset
-        // (k - 1) * 2^tb + position < 7 * 2^25 fits 32 bits
+        // the offset (k - 1) * 2^tb + position < 7 * 2^25 fits 32 bits
         g[m][k] = states[((uint32_t)(k - 1) << tb) |
-                         fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), lmask)];
+                         fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), mask)];
       }
     } else {
 #pragma unroll

@@ -232,16 +232,16 @@ CUCM_DEV void paths_issue_gathers(const GLB uint8_t *slot, uint32_t tb, uint32_t
                                   uint32_t sl[NM][8]) {
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
-    const uint32_t q = lane + 64u * m;
+    // lanes without a path repeat path 0 (its lines, in bounds)
+    const uint32_t q = lane + 64u * m < P ? lane + 64u * m : 0u;
     const uint32_t plen = 31u - clz32(q + 1u);
     const uint32_t pbits = q + 1u - (1u << plen);
     const uint32_t h0 = ((1u << plen) | pbits) & 255u;
-    const uint32_t lmask = q < P ? mask : 0u;
     uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
     sl[m][0] = h0;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
-      sl[m][k] = fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), lmask);
+      sl[m][k] = fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), mask);
     }
 #pragma unroll
     for (int k = 1; k < 8; ++k) g[m][k] = slot[((uint32_t)(k - 1) << tb) | sl[m][k]];

@@ -890,22 +890,23 @@ CUCM_DEV void paths_issue(const WaveMem &M, const GLB uint8_t *slot, uint32_t tb
   for (int m = 0; m < NM; ++m) {
     const uint32_t q = lane + 64u * m;
     const bool ok = q < P;
-    const uint32_t plen = 31u - clz32(q + 1u);
-    const uint32_t pbits = q + 1u - (1u << plen);
-    // (lanes without a path compute in-bounds garbage, and slot 0)
+    // lanes without a path repeat path 0, so their gathers fall into path
+    // 0's lines (one line per order and path per gather instruction)
+    const uint32_t qa = ok ? q : 0u;
+    const uint32_t plen = 31u - clz32(qa + 1u);
+    const uint32_t pbits = qa + 1u - (1u << plen);
     const uint32_t h0 = ((1u << (j0 + plen)) | (partial << plen) | pbits) & 255u;
-    const uint32_t lmask = ok ? mask : 0u;
     uint32_t H = (kFnvBasis ^ h0) * kFnvPrime;
     uint32_t sl[8];
     sl[0] = h0;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
-      sl[k] = fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), lmask);
+      sl[k] = fnv_pos(H, (uint32_t)((hist >> (8 * (k - 1))) & 255u), mask);
     }
     g[m][0] = M.st0[h0];
     // unconditional loads (a predicated load would be waited for at the
-    // branch join); lanes without a path all read each table's first byte.
-    // The offset (k - 1) * 2^tb + slot < 7 * 2^25 fits 32 bits.
+    // branch join).  The offset (k - 1) * 2^tb + position < 7 * 2^25 fits
+    // 32 bits.
 #pragma unroll
     for (int k = 1; k < 8; ++k) g[m][k] = slot[((uint32_t)(k - 1) << tb) | sl[k]];
     if (ok) {
