@@ -515,12 +515,12 @@ def replay_workload(args):
     # (speculation bits, resident waves, flags: 1 zero the slot per chunk, 2 no write-backs,
     # 4 pack each round's gathers into one instruction, 8 re-gather before the write-backs,
     # 16 write right after each round's gathers)
-    grid = getattr(args, "replay_grid", None) or [
-        (1, 1280, 1), (1, 2560, 1), (1, 5120, 1), (2, 1280, 1), (2, 2048, 1), (4, 1280, 1), (2, 1280, 3),
-        (1, 2560, 3), (2, 1280, 0), (1, 2560, 2), (1, 1280, 5), (1, 2560, 5), (1, 5120, 5), (2, 1280, 5),
-        (2, 2048, 5), (2, 4096, 5), (4, 1280, 5), (4, 2048, 5), (2, 2048, 7), (1, 2560, 7), (2, 1280, 9),
-        (2, 2048, 9), (1, 2560, 9), (2, 1280, 17), (1, 2560, 17), (2, 2048, 13),
-    ]
+    grid = [tuple(int(v) for v in e.split(":")) for e in args.replay_grid.split(",")] \
+        if getattr(args, "replay_grid", "") else [
+            (1, 1536, 1), (1, 3072, 1), (2, 1280, 1), (2, 1536, 1), (2, 2048, 1), (2, 3072, 1),
+            (4, 1280, 1), (2, 1536, 0), (2, 1536, 3), (1, 3072, 3), (2, 1536, 5), (2, 1536, 9),
+            (2, 1536, 17),
+        ]
     for D, waves, flags in grid:
         ms = ctypes.c_double()
         rc = probe_lib().cucm_probe_replay_decode(srca, lena, len(src), 18, D, 128, waves, flags,
