@@ -178,6 +178,8 @@ def probe_lib():
         L.cucm_probe_copy.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         L.cucm_probe_replay_decode.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32,
                                                P(ctypes.c_double)]
+        L.cucm_probe_replay_encode.argtypes = [vp, vp, u32, u32, u32, u32, u32,
+                                               P(ctypes.c_double)]
         _probe = L
     return _probe
 

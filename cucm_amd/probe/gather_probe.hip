@@ -217,6 +217,125 @@ __global__ void __launch_bounds__(256) replay_decode_kernel(const ReplayChunk *c
   if (acc == 0xdeadbeefu) atomicAdd(sink, 1u);
 }
 
+// Replay of the encoder's state traffic without its model arithmetic: per
+// unit of U bits every bit's 7 states are gathered at once (lane l takes
+// bits l + 64j, as the encoder issues them), then the 7 states of every bit
+// are written back and drained before the next unit's gathers, as the
+// encoder's deferred update does.  Lanes past the unit's bits repeat bit 0
+// (in bounds, same lines) and write nothing.
+__global__ void __launch_bounds__(256) replay_encode_kernel(const ReplayChunk *chunks, uint32_t n,
+                                                            uint32_t *counter, uint8_t *pool,
+                                                            uint64_t slot_bytes, uint32_t tb,
+                                                            uint32_t U, uint32_t flags,
+                                                            uint32_t *sink) {
+  __shared__ uint8_t ubs[4][8 + 64 + 8];  // per wave: 8 history bytes + up to 512 unit bits
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  uint8_t *ub = ubs[wid];
+  const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wid;
+  uint8_t *slot = pool + gw * slot_bytes;
+  const uint32_t mask = (1u << tb) - 1u;
+  uint32_t acc = 0, dep = 0;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= n) break;
+    const uint8_t *src = (const uint8_t *)(uintptr_t)chunks[t].src;
+    const uint64_t L = chunks[t].L;
+    if (flags & 1u) {
+      const uint64_t n16 = (7ull << tb) >> 4;
+      uint4 *d = (uint4 *)slot;
+      for (uint64_t i = lane; i < n16; i += 64) d[i] = make_uint4(0, 0, 0, 0);
+    }
+    const uint64_t vbits = 8 * L;
+    for (uint64_t base = 0; base < vbits; base += U) {
+      const uint32_t nb = (uint32_t)min((uint64_t)U, vbits - base);
+      const int64_t b0 = (int64_t)(base >> 3) - 8;
+      for (uint32_t i = lane; i < 8 + U / 8; i += 64) {
+        const int64_t p = b0 + i;
+        ub[i] = (p >= 0 && p < (int64_t)L) ? src[p] : (uint8_t)0;
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      uint32_t pos[8][7];
+      uint32_t v = 0;
+      if (flags & 4u) {
+        // 32-byte sectors: each state's whole sector is read, and written back
+        // whole with the state's byte changed (no partial-sector writes)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (64u * j < U) {
+            const uint32_t r = lane + 64u * j < nb ? lane + 64u * j : 0u;
+            const uint32_t bi = r >> 3, jb = r & 7u;
+            const uint32_t cur = ub[8 + bi];
+            const uint32_t h0 = (1u << jb) | (jb ? (cur >> (8 - jb)) : 0u);
+            uint32_t H = (2166136261u ^ h0) * 16777619u;
+            uint4 sa[7], sb[7];
+#pragma unroll
+            for (int k = 1; k < 8; ++k) {
+              const uint32_t x = H ^ (uint32_t)ub[8 + bi - k];
+              H = x * 16777619u;
+              pos[j][k - 1] = ((uint32_t)(k - 1) << tb) | ((x + dep) & mask);
+              const uint4 *sp = (const uint4 *)(slot + (pos[j][k - 1] & ~31u));
+              sa[k - 1] = sp[0];
+              sb[k - 1] = sp[1];
+            }
+#pragma unroll
+            for (int k = 0; k < 7; ++k) v += sa[k].x + sb[k].w;
+            v &= 0xffffu;
+            if (!(flags & 2u) && lane + 64u * j < nb) {
+#pragma unroll
+              for (int k = 0; k < 7; ++k) {
+                uint4 *sp = (uint4 *)(slot + (pos[j][k] & ~31u));
+                uint4 a2 = sa[k];
+                a2.x ^= v & 255u;  // (a stand-in for the changed state byte)
+                sp[0] = a2;
+                sp[1] = sb[k];
+              }
+            }
+          }
+        }
+        acc += v;
+        dep = __builtin_amdgcn_readfirstlane(v) == 0x7fffffffu ? 1u : 0u;
+        __builtin_amdgcn_s_waitcnt(0x0f70);
+        __builtin_amdgcn_wave_barrier();
+        continue;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (64u * j < U) {
+          const uint32_t r = lane + 64u * j < nb ? lane + 64u * j : 0u;
+          const uint32_t bi = r >> 3, jb = r & 7u;
+          const uint32_t cur = ub[8 + bi];
+          const uint32_t h0 = (1u << jb) | (jb ? (cur >> (8 - jb)) : 0u);
+          uint32_t H = (2166136261u ^ h0) * 16777619u;
+#pragma unroll
+          for (int k = 1; k < 8; ++k) {
+            const uint32_t x = H ^ (uint32_t)ub[8 + bi - k];
+            H = x * 16777619u;
+            pos[j][k - 1] = ((uint32_t)(k - 1) << tb) | ((x + dep) & mask);
+            v += slot[pos[j][k - 1]];
+          }
+        }
+      }
+      acc += v;
+      // the write-backs (and so the next unit's gathers) wait for this unit's
+      // gathers (v < 2^31 always, so dep stays 0)
+      dep = __builtin_amdgcn_readfirstlane(v) == 0x7fffffffu ? 1u : 0u;
+      if (!(flags & 2u)) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (64u * j < U && lane + 64u * j < nb)
+#pragma unroll
+            for (int k = 0; k < 7; ++k) slot[pos[j][k]] = (uint8_t)(v + k + dep);
+      }
+      __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the unit's writes land first
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (acc == 0xdeadbeefu) atomicAdd(sink, 1u);
+}
+
 }  // namespace
 
 // Plain streaming copy of `bytes` (a multiple of 16): the reference kernel
@@ -283,7 +402,12 @@ extern "C" int cucm_probe_replay_decode(const uint64_t *h_src, const uint32_t *h
     return -4;
   }
   ReplayChunk *h = (ReplayChunk *)malloc(sizeof(ReplayChunk) * n);
-  if (!h) return -4;
+  if (!h) {
+    (void)hipFree(d_chunks);
+    (void)hipFree(d_ctl);
+    (void)hipFree(d_pool);
+    return -4;
+  }
   for (uint32_t i = 0; i < n; ++i) h[i] = ReplayChunk{h_src[i], h_len[i], 0};
   (void)hipMemcpy(d_chunks, h, sizeof(ReplayChunk) * n, hipMemcpyHostToDevice);
   free(h);
@@ -295,6 +419,53 @@ extern "C" int cucm_probe_replay_decode(const uint64_t *h_src, const uint32_t *h
   (void)hipEventRecord(e0, 0);
   hipLaunchKernelGGL(replay_decode_kernel, dim3(waves / 4), dim3(256), 0, 0, d_chunks, n, d_ctl,
                      d_pool, slot_bytes, tb, D, U, zero_slot, d_ctl + 1);
+  (void)hipEventRecord(e1, 0);
+  hipError_t err = hipEventSynchronize(e1);
+  float f = 0;
+  (void)hipEventElapsedTime(&f, e0, e1);
+  *ms = f;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_chunks);
+  (void)hipFree(d_ctl);
+  (void)hipFree(d_pool);
+  return err == hipSuccess ? 0 : -6;
+}
+
+extern "C" int cucm_probe_replay_encode(const uint64_t *h_src, const uint32_t *h_len, uint32_t n,
+                                        uint32_t tb, uint32_t U, uint32_t waves, uint32_t flags,
+                                        double *ms) {
+  if (!h_src || !h_len || !ms || n == 0 || tb < 8 || tb > 25 || U == 0 || U > 512 || U % 8 ||
+      waves == 0 || waves % 4 || ((flags & 4u) && U > 128))
+    return -1;
+  const uint64_t slot_bytes = ((7ull << tb) + 255) & ~255ull;
+  ReplayChunk *d_chunks = nullptr;
+  uint32_t *d_ctl = nullptr;
+  uint8_t *d_pool = nullptr;
+  if (hipMalloc(&d_chunks, sizeof(ReplayChunk) * n) != hipSuccess) return -4;
+  if (hipMalloc(&d_ctl, 16) != hipSuccess || hipMalloc(&d_pool, slot_bytes * waves) != hipSuccess) {
+    (void)hipFree(d_chunks);
+    if (d_ctl) (void)hipFree(d_ctl);
+    return -4;
+  }
+  ReplayChunk *h = (ReplayChunk *)malloc(sizeof(ReplayChunk) * n);
+  if (!h) {
+    (void)hipFree(d_chunks);
+    (void)hipFree(d_ctl);
+    (void)hipFree(d_pool);
+    return -4;
+  }
+  for (uint32_t i = 0; i < n; ++i) h[i] = ReplayChunk{h_src[i], h_len[i], 0};
+  (void)hipMemcpy(d_chunks, h, sizeof(ReplayChunk) * n, hipMemcpyHostToDevice);
+  free(h);
+  (void)hipMemset(d_ctl, 0, 16);
+  (void)hipMemset(d_pool, 0, slot_bytes * waves);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, 0);
+  hipLaunchKernelGGL(replay_encode_kernel, dim3(waves / 4), dim3(256), 0, 0, d_chunks, n, d_ctl,
+                     d_pool, slot_bytes, tb, U, flags, d_ctl + 1);
   (void)hipEventRecord(e1, 0);
   hipError_t err = hipEventSynchronize(e1);
   float f = 0;

@@ -491,10 +491,11 @@ def gather_workload(args):
 
 
 def replay_workload(args):
-    """The decoder's gather stream on the uneven-size batch (chunks of at most
-    128 KiB, 2^18 slots, U = 128) replayed by libcucm_probe.so without the
-    model arithmetic, at several numbers of resident chunks and speculation
-    widths: the memory-system ceiling the decoder's access pattern allows."""
+    """The decoder's and the encoder's state traffic on the uneven-size batch
+    (chunks of at most 128 KiB, 2^18 slots, U = 128) replayed by
+    libcucm_probe.so without the model arithmetic, at several numbers of
+    resident chunks and speculation widths: the memory-system ceiling each
+    direction's access pattern allows."""
     import ctypes
     from ._native import probe_lib
     count = getattr(args, "files", 10000) or 10000
@@ -514,20 +515,26 @@ def replay_workload(args):
     out = []
     # (speculation bits, resident waves, flags: 1 zero the slot per chunk, 2 no write-backs,
     # 4 pack each round's gathers into one instruction, 8 re-gather before the write-backs,
-    # 16 write right after each round's gathers)
+    # 16 write right after each round's gathers); speculation bits 0: the encoder's stream
+    # (every gather of a unit at once, flags 1 and 2 only)
     grid = [tuple(int(v) for v in e.split(":")) for e in args.replay_grid.split(",")] \
         if getattr(args, "replay_grid", "") else [
             (1, 1536, 1), (1, 3072, 1), (2, 1280, 1), (2, 1536, 1), (2, 2048, 1), (2, 3072, 1),
             (4, 1280, 1), (2, 1536, 0), (2, 1536, 3), (1, 3072, 3), (2, 1536, 5), (2, 1536, 9),
-            (2, 1536, 17),
+            (2, 1536, 17), (0, 768, 1), (0, 1536, 1), (0, 3072, 1), (0, 1536, 3),
         ]
     for D, waves, flags in grid:
         ms = ctypes.c_double()
-        rc = probe_lib().cucm_probe_replay_decode(srca, lena, len(src), 18, D, 128, waves, flags,
-                                                  ctypes.byref(ms))
+        if D == 0:
+            rc = probe_lib().cucm_probe_replay_encode(srca, lena, len(src), 18, 128, waves, flags,
+                                                      ctypes.byref(ms))
+        else:
+            rc = probe_lib().cucm_probe_replay_decode(srca, lena, len(src), 18, D, 128, waves,
+                                                      flags, ctypes.byref(ms))
         if rc:
             raise RuntimeError("replay probe failed: %d" % rc)
-        out.append({"spec_bits": D, "waves": waves, "zero_slot": bool(flags & 1),
+        out.append({"stream": "encoder" if D == 0 else "decoder",
+                    "spec_bits": D, "waves": waves, "zero_slot": bool(flags & 1),
                     "write_back": not (flags & 18), "packed_lanes": bool(flags & 4),
                     "regather_before_write": bool(flags & 8), "write_per_round": bool(flags & 16),
                     "ms": ms.value, "MBps": total / 1e3 / ms.value})
