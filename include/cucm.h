@@ -243,12 +243,12 @@ int cucm_ctx_set_team(cucm_ctx *ctx, uint32_t mode);
 
 /* ---- serving: continuous batching ------------------------------------ */
 /* No counterpart in the reference (one file per process run there).
- * Requests of mixed kinds and sizes are queued as they arrive; each of
- * `lanes` worker lanes (own stream, device buffers and 1/lanes of the
- * context's HBM budget) takes, whenever it falls idle, the oldest pending
- * request plus every pending request of the same kind that fits its memory
- * share, and runs them as one persistent launch.  Launches of different lanes
- * overlap on the device.  A request's result never depends on its batch. */
+ * Requests of mixed kinds and sizes are queued as they arrive and admitted,
+ * oldest first, into one running kernel as soon as a job slot and memory
+ * are free: their chunk tasks join a task ring that the kernel's resident
+ * waves take as they free up, compress and decompress mixed (see
+ * cucm_serve_start).  A request's result never depends on what it ran
+ * beside. */
 #define CUCM_OP_COMPRESS 0
 #define CUCM_OP_DECOMPRESS 1
 typedef struct cucm_job {
