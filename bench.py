@@ -138,6 +138,7 @@ def stream_workload(args):
         "waves": sc["waves"],
         "compress_teams": sc["teams"],
         "decompress_teams": sd["teams"],
+        "decompress_spec_bits": sd["spec_bits"],
         "model_bytes": sc["model_bytes"],
     }
     ms_per_step = 1000.0 * t_all / args.steps
@@ -173,7 +174,7 @@ def stream_workload(args):
                                                                ", frozen model" if args.frozen
                                                                else "")),
                    "chunks": C, "chunk_size": S, "unit_bits": U, "table_bits": tb,
-                   "batch_chunks": batch, "spec_bits": args.spec, "prelearn_bytes": P,
+                   "batch_chunks": batch, "spec_bits": args.spec or "automatic", "prelearn_bytes": P,
                    "frozen": bool(args.frozen)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
@@ -299,7 +300,8 @@ def main():
                     choices=["stream", "structured", "small", "batch", "serve", "gather", "copy",
                              "replay"])
     ap.add_argument("--mib", type=int, default=0, help="input size (0 = the workload's own)")
-    ap.add_argument("--spec", type=int, default=4)
+    ap.add_argument("--spec", type=int, default=0,
+                    help="decoder bits per speculative round (0: automatic, chosen with the launch)")
     ap.add_argument("--frozen", action="store_true",
                     help="single-stream workloads: code with the frozen model")
     ap.add_argument("--prelearn-kib", type=int, default=-1,

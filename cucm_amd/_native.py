@@ -66,6 +66,7 @@ class Stats(ctypes.Structure):
         ("out_bytes", ctypes.c_uint64),
         ("model_bytes", ctypes.c_uint64),
         ("teams", ctypes.c_uint64),
+        ("spec_bits", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -18,7 +18,7 @@ MiB = 1 << 20
 
 
 def make_params(chunk_size=MiB, unit_bits=128, batch_chunks=100, prelearn_size=0, table_bits=25,
-                spec_bits=4, tail_pad=True, frozen=False):
+                spec_bits=0, tail_pad=True, frozen=False):
     """cucm_params; frozen=True selects the frozen model (CUCM_FLAG_FROZEN)."""
     p = Params()
     lib().cucm_default_params(ctypes.byref(p))
@@ -161,7 +161,7 @@ class Context:
     def compress(self, data, params=None, **kw) -> bytes:
         return self.compress_many([data], params, **kw)[0]
 
-    def decompress(self, blob, spec_bits=4) -> bytes:
+    def decompress(self, blob, spec_bits=0) -> bytes:
         return self.decompress_many([blob], spec_bits)[0]
 
     def compress_many(self, datas, params=None, **kw):
@@ -198,7 +198,7 @@ class Context:
                                       out.size, ctypes.byref(olen)), "compress")
         return out[: olen.value].tobytes()
 
-    def decompress_many(self, blobs, spec_bits=4):
+    def decompress_many(self, blobs, spec_bits=0):
         n = len(blobs)
         bufs = [_buf(b) for b in blobs]
         ins = (ctypes.c_void_p * n)(*[b[0] for b in bufs])
@@ -237,7 +237,7 @@ class Context:
             check(rc, "compress_batch")
             return out[: olen.value].tobytes()
 
-    def decompress_batch(self, blob, n, chunk_size, unit_bits=128, table_bits=25, spec_bits=4) -> bytes:
+    def decompress_batch(self, blob, n, chunk_size, unit_bits=128, table_bits=25, spec_bits=0) -> bytes:
         addr, bl, keep = _buf(blob)
         out = np.empty(n, dtype=np.uint8)
         check(lib().cucm_decompress_batch(self.handle, addr, bl, out.ctypes.data, n, chunk_size,
@@ -270,7 +270,7 @@ class Context:
                                                  cap, ctypes.byref(olen)), "compress_noalloc")
         return olen.value
 
-    def decompress_many_device(self, ins, lens, outs, caps, spec_bits=4, plain=False):
+    def decompress_many_device(self, ins, lens, outs, caps, spec_bits=0, plain=False):
         n = len(ins)
         a = (ctypes.c_void_p * n)(*ins)
         l = (ctypes.c_uint64 * n)(*lens)
@@ -318,7 +318,7 @@ class Context:
             lib().cucm_stream_end(s)
         return b"".join(parts)
 
-    def decompress_stream(self, blob, spec_bits=4, records_per_call=1):
+    def decompress_stream(self, blob, spec_bits=0, records_per_call=1):
         """Decompresses through the streaming C ABI, `records_per_call` batch
         records per call."""
         blob = bytes(blob)
@@ -384,7 +384,7 @@ class Context:
     def submit_compress(self, data, params) -> int:
         return self._submit(0, data, params, output_cap(len(data), params))
 
-    def submit_decompress(self, blob, spec_bits=4) -> int:
+    def submit_decompress(self, blob, spec_bits=0) -> int:
         size = int.from_bytes(bytes(blob[:4]), "big") if len(blob) >= 4 else 0
         return self._submit(1, blob, make_params(spec_bits=spec_bits), size)
 
@@ -420,7 +420,7 @@ def compress(data, chunk_size=MiB, unit_bits=128, batch_chunks=100, prelearn_siz
     return default_context().compress(data, p)
 
 
-def decompress(blob, spec_bits=4) -> bytes:
+def decompress(blob, spec_bits=0) -> bytes:
     return default_context().decompress(blob, spec_bits)
 
 

@@ -109,9 +109,10 @@ struct Shape {
   uint32_t slots() const { return team ? blocks : blocks * wpb; }
 };
 // Shape of the unit codec's launch within mem_left bytes of model slots of
-// slot_bytes each (no slot limit when slot_bytes is 0).
+// slot_bytes each (no slot limit when slot_bytes is 0); one_wave rules out
+// teams.
 Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spec, bool dec,
-                 uint64_t slot_bytes, uint64_t mem_left);
+                 uint64_t slot_bytes, uint64_t mem_left, bool one_wave = false);
 // Shape of the frozen-model codec's launch (no model slots).
 Shape frozen_shape(const cucm_ctx *c, uint64_t ntasks);
 // Tasks of one codec pass by kind: the unit codec's (U <= 256), the frozen
@@ -137,10 +138,12 @@ struct PassCounts {
 struct PassPlan {
   Shape sh, fz, bg;  // unit codec, frozen codec, large-unit codec
   uint32_t Umax = 32, Ubig = 0;
+  uint32_t spec = 0;  // decoder bits per round (spec 0 resolved with the shape)
   uint64_t slot_bytes = 0, big_bytes = 0;
 };
 Shape big_shape(const cucm_ctx *c, uint64_t ntasks, uint64_t per_wave, uint64_t mem_left);
-// Shapes within mem_left bytes, model slots and large-unit records reserved.
+// Shapes within mem_left bytes, model slots and large-unit records reserved;
+// a decoder pass with spec 0 also chooses its speculation width (pl.spec).
 int plan_pass(cucm_ctx *c, const PassCounts &n, uint32_t spec, bool dec, uint64_t mem_left,
               PassPlan &pl, cucm_stats &st);
 // The plan's buffers in a's pool / slot / large-unit fields.

@@ -59,8 +59,11 @@ typedef struct cucm_params {
   uint16_t batch_chunks;  /* chunks per batch record (reference -b, default 100) */
   uint32_t prelearn_size; /* P, leading bytes that train the initial model (-p) */
   uint8_t table_bits;     /* log2 slots per order table, 8..25; 25 = reference */
-  uint8_t spec_bits;      /* decoder bits per speculative round: 1, 2, 4 or 8
-                             (0 = default 4); never changes the stream */
+  uint8_t spec_bits;      /* decoder bits per speculative round: 1, 2, 4 or 8;
+                             0 (default) = automatic: 4-bit rounds as teams while
+                             every chunk of a launch fits resident as a team,
+                             else one wave per chunk with 2-bit rounds; never
+                             changes the stream */
   uint16_t reserved;
   uint32_t flags;         /* CUCM_FLAG_*; other bits are rejected (CUCM_E_PARAM) */
 } cucm_params;
@@ -209,6 +212,7 @@ typedef struct cucm_stats {
   uint64_t in_bytes, out_bytes;
   uint64_t model_bytes;    /* HBM held by model slots */
   uint64_t teams;          /* workgroups that ran as teams of 4 waves (0: one wave per chunk) */
+  uint64_t spec_bits;      /* decoder bits per speculative round of the last decode launch */
 } cucm_stats;
 int cucm_ctx_last_stats(cucm_ctx *ctx, cucm_stats *st);
 
@@ -236,9 +240,10 @@ int cucm_ctx_set_waves_per_block(cucm_ctx *ctx, uint32_t waves);
  * speculative round chain; the others record rounds, resolve states, walk
  * the StateMap recurrences and hash the next round's paths.  mode 0
  * (default): teams when every chunk of a launch can be resident as a team at
- * once (decoder: spec_bits 1, 2 or 4); 1: never; 2: whenever allowed.  Never
- * changes the output.  Automatic mode also uses teams when the chunks fill
- * at least two full passes of resident teams. */
+ * once (decoder: spec_bits 1, 2 or 4; with spec_bits 0 only then); 1: never;
+ * 2: whenever allowed.  Never changes the output.  Automatic mode also uses
+ * teams when the chunks fill at least two full passes of resident teams
+ * (encoder, and decoder with a fixed spec_bits). */
 int cucm_ctx_set_team(cucm_ctx *ctx, uint32_t mode);
 
 /* ---- serving: continuous batching ------------------------------------ */

@@ -3,8 +3,8 @@ cucm_team.h) on the GPU: containers byte-identical to the oracle's and
 round trips lossless on every configuration and speculation width they
 support, with a prelearned model, with more chunks than resident teams
 (workgroups reuse their LDS and model slot from chunk to chunk), over many
-containers in one launch, and chosen automatically for a launch whose chunks
-all fit."""
+containers in one launch, chosen automatically for a launch whose chunks
+all fit, and the automatic speculation width."""
 import numpy as np
 import pytest
 
@@ -113,5 +113,33 @@ def test_auto_mode_runs_few_chunks_as_teams(tctx, oracle):
             # speculation bits; one wave per chunk otherwise
             want = 20 if (mode == 0 and spec <= 4) else 5
             assert tctx.stats()["waves"] == want
+        finally:
+            tctx.set_team(0)
+
+
+def test_auto_speculation_width_follows_residency(tctx, oracle):
+    """spec_bits 0: 4-bit rounds as teams while every chunk fits resident as
+    a team; with more chunks (here 1,200 of 256 B against at most 2 teams
+    per CU), one wave per chunk with 2-bit rounds.  Forced team modes keep
+    their execution.  The output never changes."""
+    few = _data("structured", 40000, 9)
+    pf = make_params(8192, 128, 100, 0, 16)
+    bf = tctx.compress(few, pf)
+    assert tctx.decompress(bf, 0) == few
+    st = tctx.stats()
+    assert (st["spec_bits"], st["waves"]) == (4, 20)
+    many = _data("markov2", 1200 * 256, 12)
+    pm = make_params(256, 128, 100, 0, 10)
+    bm = tctx.compress(many, pm)
+    assert bm == oracle.compress(many, 256, 128, 100, 0, 10)
+    assert tctx.decompress(bm, 0) == many
+    st = tctx.stats()
+    assert st["spec_bits"] == 2 and st["teams"] == 0 and st["tasks"] == 1200
+    for mode, want in ((1, (2, 0)), (2, (4, 1))):
+        tctx.set_team(mode)
+        try:
+            assert tctx.decompress(bm, 0) == many
+            st = tctx.stats()
+            assert (st["spec_bits"], min(st["teams"], 1)) == want
         finally:
             tctx.set_team(0)
