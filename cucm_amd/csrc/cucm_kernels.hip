@@ -31,7 +31,10 @@ __global__ void __launch_bounds__(512) codec_kernel(CodecArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   LDS DevTables *T = (LDS DevTables *)lds;
   stage_tables(a.tabs, T);
-  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  // (the wave index is uniform; readfirstlane lets the compiler keep every
+  // per-wave base address in scalar registers)
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
   constexpr uint32_t P = kDecode ? (1u << D) - 1u : 0u;
   const uint32_t wbytes = wave_lds_bytes(Umax, P, kDecode);
   LDS uint8_t *wbase = (LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * wbytes;
@@ -110,7 +113,10 @@ __global__ void __launch_bounds__(512) frozen_kernel(CodecArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   LDS DevTables *T = (LDS DevTables *)lds;
   stage_tables(a.tabs, T);
-  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  // (the wave index is uniform; readfirstlane lets the compiler keep every
+  // per-wave base address in scalar registers)
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
   const FrozenMem F =
       carve_frozen((LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * frozen_wave_bytes());
   uint32_t loaded = kNoModel;
@@ -136,7 +142,10 @@ __global__ void __launch_bounds__(256) bigunit_kernel(CodecArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   LDS DevTables *T = (LDS DevTables *)lds;
   stage_tables(a.tabs, T);
-  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  // (the wave index is uniform; readfirstlane lets the compiler keep every
+  // per-wave base address in scalar registers)
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
   const BigMem M =
       carve_big((LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * big_wave_bytes());
   const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wid;
@@ -245,7 +254,10 @@ __global__ void __launch_bounds__(512) serve_kernel(ServeArgs s) {
   extern __shared__ __align__(16) uint8_t lds[];
   LDS DevTables *T = (LDS DevTables *)lds;
   stage_tables(s.a.tabs, T);
-  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  // (the wave index is uniform; readfirstlane lets the compiler keep every
+  // per-wave base address in scalar registers)
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wbytes = wave_lds_bytes(Umax, P, true);
   const WaveMem M = carve((LDS uint8_t *)lds + align16(sizeof(DevTables)) + wid * wbytes, Umax, P,
                           true);
