@@ -235,8 +235,14 @@ Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spe
     s.blocks = (uint32_t)blocks;
     return s;
   }
+  // Decoding with model slots of 8 MiB or more (2^20 slots and up), at most
+  // six waves per CU: the seventh that LDS allows at U = 128 speeds the
+  // 10,000-file batch up (2^18: 229.9 -> 243.9 MB/s) but slows 4096 x 256
+  // KiB chunks at 2^22 down (218.4 -> 207.8 MB/s), where the memory system
+  // saturates with fewer chunks in flight.
+  const uint32_t wcap = (dec && slot_bytes >= (8ull << 20)) ? 6u : 8u;
   uint32_t wmax = 1;
-  for (uint32_t w = 8; w >= 1; --w)
+  for (uint32_t w = wcap; w >= 1; --w)
     if (codec_lds_bytes(Umax, spec, dec, w) <= c->lds_block) {
       wmax = w;
       break;

@@ -168,11 +168,21 @@ namespace cucm {
 constexpr uint32_t kEmpty = 0xffffffffu;
 constexpr uint16_t kNoState = 0x100;
 
+// Entries per order of a unit's distinct-slot table: teams use a power of
+// two of at least 2U; one wave per chunk uses 1.5U (96, 192 or 384), at
+// most two thirds full, so its LDS holds a seventh wave per CU at U = 128.
 CUCM_HD uint32_t table_size(uint32_t U) {
   uint32_t t = 64;
   while (t < 2 * U) t <<= 1;
   return t;
 }
+CUCM_HD uint32_t wave_table_size(uint32_t U) { return U * 3u / 2u; }
+// Home entry of a table position among ts entries (ts <= 2^16): the high
+// half of a multiplicative hash, scaled to ts; probes move to the next entry.
+CUCM_HDI uint32_t table_home(uint32_t pos, uint32_t ts) {
+  return (((pos * 0x9E3779B1u) >> 16) * ts) >> 16;
+}
+CUCM_HDI uint32_t table_next(uint32_t h, uint32_t ts) { return h + 1u == ts ? 0u : h + 1u; }
 
 CUCM_HD uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
@@ -186,7 +196,8 @@ struct WaveMem {
   LDS uint16_t *tval;   // teams: [7][TS] running state of each distinct slot (kNoState: untouched)
   LDS uint8_t *tv8;     // one wave per chunk: [7][TS] running state of each distinct slot,
                         // set to the unit-start state by every bit that records the slot
-  LDS uint16_t *ridx;   // [8][Umax] table index (order 0: h0) per unit bit
+  LDS uint16_t *ridx;   // [8][Umax] table index (order 0: h0) per unit bit (16-bit)
+  LDS uint8_t *rix8;    // the same as bytes (one wave per chunk, Umax <= 128)
   LDS uint16_t *rprm;   // [Umax] pr << 1 | y per unit bit (persists: tail padding)
   LDS uint8_t *st0;     // [256] order-0 states by h0
   LDS uint8_t *rg;      // [8][Umax] unit-start state per unit bit
@@ -194,19 +205,23 @@ struct WaveMem {
   LDS uint8_t *pg;      // [2][P][8] decoder: unit-start state per path, two rounds
   LDS uint8_t *ub;      // [8 + Umax/8] 8 history bytes + the unit's bytes
   LDS uint32_t *dummy;  // [64] scratch target of predicated-off LDS operations
+  uint32_t ts;          // entries per order of tkey / tval / tv8
 };
 
 // Byte offsets of the per-wave LDS arrays (16-byte aligned each); the
 // decoder keeps nbuf rounds of path slots and states.  One wave per chunk
-// keeps the distinct slots' running states as bytes (tv8), teams as 16-bit
-// words with an untouched marker (tval, wide).
+// keeps the distinct slots' running states as bytes (tv8) in a table of
+// wave_table_size entries per order and, up to 128-bit units, its per-bit
+// table indices as bytes (rix8); teams (wide) keep 16-bit states with an
+// untouched marker (tval), power-of-two tables and 16-bit indices.
 struct WaveLayout {
   uint32_t stp, w, tkey, pslot, tval, ridx, rprm, st0, rg, rcxt, pg, ub, dummy, total;
 };
 
 CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec, uint32_t nbuf = 2,
                                 bool wide = false) {
-  const uint32_t TS = table_size(Umax);
+  const uint32_t TS = wide ? table_size(Umax) : wave_table_size(Umax);
+  const bool idx8 = !wide && Umax <= 128u;
   const uint32_t Pd = dec ? P : 0u;
   WaveLayout l;
   uint32_t off = 0;
@@ -218,7 +233,7 @@ CUCM_HDI WaveLayout wave_layout(uint32_t Umax, uint32_t P, bool dec, uint32_t nb
   CUCM_TAKE(tkey, 7 * TS * 4)
   CUCM_TAKE(pslot, nbuf * Pd * 8 * 4)
   CUCM_TAKE(tval, 7 * TS * (wide ? 2 : 1))
-  CUCM_TAKE(ridx, 8 * Umax * 2)
+  CUCM_TAKE(ridx, 8 * Umax * (idx8 ? 1 : 2))
   CUCM_TAKE(rprm, Umax * 2)
   CUCM_TAKE(st0, 256)
   CUCM_TAKE(rg, 8 * Umax)
@@ -245,7 +260,10 @@ CUCM_DEV WaveMem carve(LDS uint8_t *base, uint32_t Umax, uint32_t P, bool dec,
   m.pslot = (LDS uint32_t *)(base + l.pslot);
   m.tval = wide ? (LDS uint16_t *)(base + l.tval) : nullptr;
   m.tv8 = wide ? nullptr : base + l.tval;
-  m.ridx = (LDS uint16_t *)(base + l.ridx);
+  const bool idx8 = !wide && Umax <= 128u;
+  m.ridx = idx8 ? nullptr : (LDS uint16_t *)(base + l.ridx);
+  m.rix8 = idx8 ? base + l.ridx : nullptr;
+  m.ts = wide ? table_size(Umax) : wave_table_size(Umax);
   m.rprm = (LDS uint16_t *)(base + l.rprm);
   m.st0 = base + l.st0;
   m.rg = base + l.rg;
@@ -254,6 +272,23 @@ CUCM_DEV WaveMem carve(LDS uint8_t *base, uint32_t Umax, uint32_t P, bool dec,
   m.ub = base + l.ub;
   m.dummy = (LDS uint32_t *)(base + l.dummy);
   return m;
+}
+
+// Per-bit table index of the one-wave kernels: bytes up to 128-bit units
+// (NJ <= 2), 16-bit words above.
+template <int NJ>
+CUCM_DEV uint32_t rix_get(const WaveMem &M, uint32_t i) {
+  if constexpr (NJ <= 2)
+    return M.rix8[i];
+  else
+    return M.ridx[i];
+}
+template <int NJ>
+CUCM_DEV void rix_set(const WaveMem &M, uint32_t i, uint32_t v) {
+  if constexpr (NJ <= 2)
+    M.rix8[i] = (uint8_t)v;
+  else
+    M.ridx[i] = (uint16_t)v;
 }
 
 // Per-phase shader-clock totals of one wave (enabled when args.prof != null):
@@ -358,8 +393,7 @@ CUCM_DEV void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M, G
     for (uint32_t i = lane; i < kMixCtx * kOrders; i += 64) M.w[i] = 0;
     for (uint32_t i = lane; i < 256; i += 64) M.st0[i] = 0;
   }
-  const uint32_t TS = table_size(Umax);
-  for (uint32_t i = lane; i < 7 * TS; i += 64) M.tkey[i] = kEmpty;
+  for (uint32_t i = lane; i < 7 * M.ts; i += 64) M.tkey[i] = kEmpty;
   for (uint32_t i = lane; i < Umax; i += 64) M.rprm[i] = 0;
   for (uint32_t i = lane; i < 8 + Umax / 8 + 8; i += 64) M.ub[i] = 0;
   vm_drain();
@@ -383,7 +417,7 @@ template <int NJ>
 CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_t *slot,
                             uint32_t tb, uint32_t Umax, uint32_t nvalid, uint32_t lane,
                             PhaseClock &clk, uint64_t ym[NJ]) {
-  const uint32_t TS = table_size(Umax);
+  const uint32_t TS = M.ts;
   const uint32_t nbytes = nvalid >> 3;
   // y of every unit bit as a uniform bit mask
 #pragma unroll
@@ -414,9 +448,16 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
     bool coll = false;
     if (it < 7 * nbytes) {
       const uint32_t b = it / 7, k = 1 + it % 7;
-      const u32x4 v = *(const LDS u32x4 *)(M.ridx + k * Umax + 8 * b);
-      const uint32_t e[8] = {v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16,
-                             v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16};
+      uint32_t e[8];
+      if constexpr (NJ <= 2) {
+        const uint64_t v = *(const LDS uint64_t *)(M.rix8 + k * Umax + 8 * b);
+#pragma unroll
+        for (int x = 0; x < 8; ++x) e[x] = (uint32_t)(v >> (8 * x)) & 255u;
+      } else {
+        const u32x4 v = *(const LDS u32x4 *)(M.ridx + k * Umax + 8 * b);
+        e[0] = v.x & 0xffffu, e[1] = v.x >> 16, e[2] = v.y & 0xffffu, e[3] = v.y >> 16;
+        e[4] = v.z & 0xffffu, e[5] = v.z >> 16, e[6] = v.w & 0xffffu, e[7] = v.w >> 16;
+      }
 #pragma unroll
       for (int x = 0; x < 8; ++x)
 #pragma unroll
@@ -442,8 +483,7 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
       for (uint32_t b = 8u * seg; b < bend; ++b) {
         const uint32_t r = 8 * b + j;
         const uint32_t y = (uint32_t)(yseg >> (r & 63)) & 1u;
-        const uint32_t i = M.ridx[k * Umax + r];
-        const uint32_t g = M.rg[k * Umax + r];
+        const uint32_t i = rix_get<NJ>(M, k * Umax + r);
         const uint32_t reps = ((cmask >> b) & 1u) ? 8u : 1u;
         for (uint32_t q = 0; q < reps; ++q) {
           if (reps == 1u || q == j) {
@@ -468,7 +508,7 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
   for (uint32_t k = 1; k < 8; ++k) {
     GLB uint8_t *tab = slot + ((uint64_t)(k - 1) << tb);
     for (uint32_t r = lane; r < nvalid; r += 64) {
-      const uint32_t i = M.ridx[k * Umax + r];
+      const uint32_t i = rix_get<NJ>(M, k * Umax + r);
       tab[M.tkey[(k - 1) * TS + i]] = M.tv8[(k - 1) * TS + i];
     }
   }
@@ -476,7 +516,7 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
 #pragma unroll 1
   for (uint32_t k = 1; k < 8; ++k)
     for (uint32_t r = lane; r < nvalid; r += 64) {
-      const uint32_t i = M.ridx[k * Umax + r];
+      const uint32_t i = rix_get<NJ>(M, k * Umax + r);
       M.tkey[(k - 1) * TS + i] = kEmpty;
     }
   vm_drain();
@@ -617,7 +657,7 @@ CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
                                              uint32_t Umax, uint32_t lane, uint32_t sl[NJ][8],
                                              uint32_t g[NJ][8], uint32_t idx[NJ][8],
                                              uint32_t r0 = 0) {
-  const uint32_t TS = table_size(Umax);
+  const uint32_t TS = M.ts;
   // 8 history bytes and the unit's bytes (zero outside the chunk)
   const int64_t b0 = (int64_t)(base >> 3) - 8;
   for (uint32_t i = lane; i < 8 + U / 8; i += 64) {
@@ -650,7 +690,7 @@ CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
     const bool ok = r0 + lane + 64u * j < nvalid;
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
-      idx[j][k] = sl[j][k] & (TS - 1);
+      idx[j][k] = table_home(sl[j][k], TS);
       LDS uint32_t *at = ok ? &M.tkey[(k - 1) * TS + idx[j][k]] : &M.dummy[lane];
       old[j][k] = lds_cas(at, kEmpty, sl[j][k]);
     }
@@ -668,7 +708,7 @@ CUCM_DEV void encode_issue(const WaveMem &M, const GLB uint8_t *src, uint64_t L,
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
         const bool pb = (pend >> (j * 8 + k)) & 1u;
-        idx[j][k] = pb ? ((idx[j][k] + 1) & (TS - 1)) : idx[j][k];
+        idx[j][k] = pb ? table_next(idx[j][k], TS) : idx[j][k];
         LDS uint32_t *at = pb ? &M.tkey[(k - 1) * TS + idx[j][k]] : &M.dummy[lane];
         old[j][k] = lds_cas(at, kEmpty, sl[j][k]);
       }
@@ -689,7 +729,7 @@ CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
                                                uint32_t nvalid, uint32_t Umax, uint32_t lane,
                                                const uint32_t sl[NJ][8], const uint32_t g[NJ][8],
                                                const uint32_t idx[NJ][8]) {
-  const uint32_t TS = table_size(Umax);
+  const uint32_t TS = M.ts;
   uint32_t pv[NJ][8];
 #pragma unroll
   for (int j = 0; j < NJ; ++j)
@@ -724,11 +764,11 @@ CUCM_DEV void encode_predict(const WaveMem &M, const LDS DevTables *T,
     if (ok) {
       M.rprm[rr] = (uint16_t)((pr << 1) | y);
       M.rcxt[rr] = (uint8_t)cxt[j];
-      M.ridx[rr] = (uint16_t)sl[j][0];
+      rix_set<NJ>(M, rr, sl[j][0]);
       M.rg[rr] = (uint8_t)g[j][0];
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
-        M.ridx[k * Umax + rr] = (uint16_t)idx[j][k];
+        rix_set<NJ>(M, k * Umax + rr, idx[j][k]);
         M.rg[k * Umax + rr] = (uint8_t)g[j][k];
         // every bit of a slot gathered the same unit-start state
         M.tv8[(k - 1) * TS + idx[j][k]] = (uint8_t)g[j][k];
@@ -975,7 +1015,7 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
   constexpr uint32_t P = (1u << D) - 1u;
   constexpr int NM = (int)((P + 63u) / 64u);
   const uint32_t U = tk.U, tb = tk.tb, mask = (1u << tb) - 1u;
-  const uint32_t TS = table_size(Umax);
+  const uint32_t TS = M.ts;
   const uint64_t L = tk.L;
   const GLB uint8_t *pay = (const GLB uint8_t *)(uintptr_t)tk.src;
   const uint64_t plen = tk.aux;
@@ -1061,19 +1101,19 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
         const uint32_t gq = pgb[q * 8 + k];
         const uint32_t sl = psb[q * 8 + k];
         const bool ins = rec && k != 0;
-        uint32_t h = sl & (TS - 1);
+        uint32_t h = table_home(sl, TS);
         LDS uint32_t *tkk = M.tkey + (k ? (k - 1) * TS : 0);
         uint32_t old = lds_cas((ins ? &tkk[h] : &M.dummy[lane]), kEmpty, sl);
         if (wave_ballot(ins && old != kEmpty && old != sl)) {
           if (ins)
             while (old != kEmpty && old != sl) {
-              h = (h + 1) & (TS - 1);
+              h = table_next(h, TS);
               old = lds_cas(&tkk[h], kEmpty, sl);
             }
         }
         if (rec) {
           M.rg[k * Umax + r] = (uint8_t)gq;
-          M.ridx[k * Umax + r] = (uint16_t)(k == 0 ? sl : h);
+          rix_set<NJ>(M, k * Umax + r, k == 0 ? sl : h);
         }
         // the slot's unit-start state (every round of the unit gathered the same)
         if (ins) M.tv8[(k - 1) * TS + h] = (uint8_t)gq;
