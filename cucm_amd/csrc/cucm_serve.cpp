@@ -261,8 +261,9 @@ int Server::init_engine(uint32_t max_jobs) {
   slot_bytes = up256(7ull << kServeTbMax);
   // workgroups of 4 waves, one per SIMD: a fifth wave on a CU shares a SIMD
   // with an older wave and runs its task several times slower, and a
-  // request's latency is that of its slowest task
-  uint32_t wmax = 4;
+  // request's latency is that of its slowest task (cucm_ctx_set_waves_per_block
+  // overrides it, up to what LDS holds)
+  uint32_t wmax = c->waves_per_block ? std::min<uint32_t>(c->waves_per_block, 8u) : 4u;
   while (wmax > 1 && serve_lds_bytes(wmax) > c->lds_block) --wmax;
   const uint64_t per_cu = std::max<uint64_t>(1, c->lds_cu / serve_lds_bytes(wmax));
   uint64_t waves = (uint64_t)c->cus * per_cu * wmax;

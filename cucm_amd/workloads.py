@@ -259,7 +259,12 @@ def batch_cpu_baseline(arena, offs, sizes, d_c, coffs, cl, s_max, tb, args):
     from .autotune import stratified_sample
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
         (os.cpu_count() or 1)
-    out = {"unit": "MB/s (compress)", "kind": "port", "cores": threads}
+    # (on the GPU box OMP_NUM_THREADS is the host share of one GPU, 16 threads;
+    # os.cpu_count() reports the whole host)
+    out = {"unit": "MB/s (compress)", "kind": "port", "cores": threads,
+           "host_cpu_count": os.cpu_count(),
+           "threads_source": "--cpu-threads" if args.cpu_threads else
+           ("OMP_NUM_THREADS" if os.environ.get("OMP_NUM_THREADS") else "os.cpu_count()")}
     same = True
     for label, nthr, budget in (("all", threads, args.cpu_mib * MiB),
                                 ("one", 1, max(args.cpu_mib * MiB // 12, 1 << 20))):
