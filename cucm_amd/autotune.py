@@ -84,6 +84,7 @@ def _run(ctx, ins, lens, params, cs, caps, outs, direction, launch, cl=None):
     (kernel ms, compressed lengths)."""
     ctx.set_waves_per_block(launch["waves_per_block"])
     ctx.set_team(2 if launch["team"] else 1)
+    ctx.set_max_waves(launch.get("max_waves", 0))
     if direction == "encoder":
         cl, st = ctx.compress_many_device(ins, lens, params, cs, caps)
     else:
@@ -93,10 +94,17 @@ def _run(ctx, ins, lens, params, cs, caps, outs, direction, launch, cl=None):
     return ctx.stats()["codec_ms"], cl
 
 
+# max_waves caps the resident waves (teams: 4 per chunk): on the 10,000-file
+# batch the encoder's teams run faster at 2.5 per CU than at the 3 LDS allows
+# (545.8 against 516.4 MB/s), since the memory system serves fewer chunks in
+# flight better
+RESIDENCY_GRID = (2048, 2560)
 ENCODER_LAUNCH = [{"team": True, "waves_per_block": 0}] + \
+    [{"team": True, "waves_per_block": 0, "max_waves": m} for m in RESIDENCY_GRID] + \
     [{"team": False, "waves_per_block": w} for w in WPB_GRID]
 DECODER_LAUNCH = [{"team": True, "waves_per_block": 0, "spec_bits": d} for d in TEAM_SPEC_GRID] + \
-    [{"team": False, "waves_per_block": w, "spec_bits": d} for d in SPEC_GRID for w in WPB_GRID]
+    [{"team": False, "waves_per_block": w, "spec_bits": d} for d in SPEC_GRID for w in WPB_GRID] + \
+    [{"team": False, "waves_per_block": 0, "spec_bits": 2, "max_waves": 1536}]
 
 
 def autotune_batch(ctx, arena, offs, sizes, sample_bytes=640 << 20, ratio_tol=0.01, log=None,
@@ -154,10 +162,11 @@ def autotune_batch(ctx, arena, offs, sizes, sample_bytes=640 << 20, ratio_tol=0.
     dec = min(dec_trials, key=lambda t: t["kernel_ms"])
     ctx.set_waves_per_block(0)
     ctx.set_team(0)
+    ctx.set_max_waves(0)
+    keys = ("team", "waves_per_block", "max_waves", "spec_bits")
     res = {"chunk_max": best["chunk_max"], "table_bits": best["table_bits"],
-           "encoder": {"team": enc["team"], "waves_per_block": enc["waves_per_block"]},
-           "decoder": {"team": dec["team"], "waves_per_block": dec["waves_per_block"],
-                       "spec_bits": dec["spec_bits"]},
+           "encoder": {k: enc[k] for k in keys if k in enc},
+           "decoder": {k: dec[k] for k in keys if k in dec},
            "sample_files": len(idx), "sample_bytes": int(total), "ratio_tol": ratio_tol,
            "seconds": time.time() - t0, "stream_trials": trials,
            "encoder_trials": enc_trials, "decoder_trials": dec_trials}
@@ -172,8 +181,10 @@ def autotune_batch(ctx, arena, offs, sizes, sample_bytes=640 << 20, ratio_tol=0.
 
 def apply_launch(ctx, launch):
     """Sets a direction's launch settings (team: True always, False never,
-    None automatic; waves per workgroup, 0 automatic) on ctx."""
+    None automatic; waves per workgroup, 0 automatic; resident waves at most
+    max_waves, 0 as many as fit) on ctx."""
     ctx.set_waves_per_block(launch.get("waves_per_block", 0))
+    ctx.set_max_waves(launch.get("max_waves", 0))
     team = launch.get("team")
     ctx.set_team(0 if team is None else (2 if team else 1))
 

@@ -3,9 +3,10 @@
 // Requests (compress or decompress of caller-owned host buffers) enter a FIFO
 // through cucm_submit.  A scheduler thread admits them, oldest first, as soon
 // as a job slot, room in the task ring and arena memory are free: it copies
-// the request into host memory shared with the device, builds its chunk
-// tasks on the host (headers, prelearned models and batch records are parsed
-// there) and publishes them by advancing the ring's tail.  One persistent
+// the request into pinned host memory and from there into device memory,
+// builds its chunk tasks on the host (headers, prelearned models and batch
+// records are parsed there) and publishes them by advancing the ring's
+// tail.  One persistent
 // launch of serve_kernel (cucm_kernels.hip) takes the tasks of every admitted
 // request, compress and decompress mixed, as its waves free up, so a request
 // that arrives while others are on the GPU starts on the next free wave
@@ -142,7 +143,7 @@ struct SJob {
   // serving kernel
   int slot = -1;
   uint32_t gen = 0, ntasks = 0;
-  uint64_t h_off = 0, h_len = 0, d_off = 0, d_len = 0;
+  uint64_t h_off = 0, h_len = 0, d_off = 0, d_len = 0, i_off = 0, i_len = 0;
   uint64_t out_off = 0;  // container / decoded bytes in the shared arena
 };
 
@@ -189,8 +190,9 @@ struct Server {
   uint64_t *lens = nullptr;
   DevInitModel *models = nullptr;
   uint8_t *hbase = nullptr;
-  DevBuf pool, results, head, jobcount, staging;
-  Arena harena, darena;
+  DevBuf pool, results, head, jobcount, staging, dinput;
+  Arena harena, darena, iarena;
+  hipStream_t cs = nullptr;  // copies of admitted requests' inputs to the device
   Shape shape;
   uint64_t slot_bytes = 0;
   bool engine_ok = false;
@@ -219,6 +221,7 @@ struct Server {
     }
 
     if (ks) (void)hipStreamDestroy(ks);
+    if (cs) (void)hipStreamDestroy(cs);
   }
 
   int init_engine(uint32_t max_jobs);
@@ -237,7 +240,9 @@ int Server::init_engine(uint32_t max_jobs) {
   cucm_ctx *c = parent;
   J = max_jobs;
   if (hipSetDevice(c->device) != hipSuccess) return CUCM_E_DEVICE;
-  if (hipStreamCreateWithFlags(&ks, hipStreamNonBlocking) != hipSuccess) return CUCM_E_DEVICE;
+  if (hipStreamCreateWithFlags(&ks, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+    return CUCM_E_DEVICE;
   int rc;
   if ((rc = ctl_mem.alloc(sizeof(Ctl))) || (rc = ring_mem.alloc(sizeof(Task) * kRing)) ||
       (rc = jobs_mem.alloc(sizeof(ServeJob) * J)) || (rc = done_mem.alloc(4ull * J)) ||
@@ -257,6 +262,11 @@ int Server::init_engine(uint32_t max_jobs) {
   harena.init(arena);
   if ((rc = staging.reserve(arena))) return rc;
   darena.init(arena);
+  // and as much again for the requests' inputs (and prelearned models) on
+  // the device, copied there at admission: read across the host link, a
+  // decoder's payload window and an encoder's unit bytes stall every wave
+  if ((rc = dinput.reserve(arena))) return rc;
+  iarena.init(arena);
   // the kernel's resident waves: as LDS allows, within the rest of the budget
   slot_bytes = up256(7ull << kServeTbMax);
   // workgroups of 4 waves, one per SIMD: a fifth wave on a CU shares a SIMD
@@ -310,7 +320,8 @@ void Server::complete(SJob &j, int32_t status, uint64_t out_len) {
 void Server::release(SJob &j) {
   if (j.h_len) harena.release(j.h_off, j.h_len);
   if (j.d_len) darena.release(j.d_off, j.d_len);
-  j.h_len = j.d_len = 0;
+  if (j.i_len) iarena.release(j.i_off, j.i_len);
+  j.h_len = j.d_len = j.i_len = 0;
   if (j.slot >= 0) {
     slot_job[j.slot] = 0;
     free_slots.push_back((uint32_t)j.slot);
@@ -432,17 +443,27 @@ int Server::admit(uint64_t id, SJob &j) {
   const uint64_t hneed = st_off + up256(states);
   // device staging: compress payloads, or decompress chunks' decoded bytes
   const uint64_t dneed = q.op == CUCM_OP_COMPRESS ? C * cap : C * S;
-  if (hneed > harena.cap || dneed > darena.cap) return CUCM_E_UNSUPPORTED;
-  uint64_t hoff = 0, doff = 0;
+  // the input and the model's states, on the device
+  const uint64_t ineed = up256(q.in_len) + up256(states);
+  if (hneed > harena.cap || dneed > darena.cap || ineed > iarena.cap) return CUCM_E_UNSUPPORTED;
+  uint64_t hoff = 0, doff = 0, ioff = 0;
   if (!harena.alloc(hneed, hoff)) return CUCM_E_NOTREADY;
   if (dneed && !darena.alloc(dneed, doff)) {
     harena.release(hoff, hneed);
+    return CUCM_E_NOTREADY;
+  }
+  if (!iarena.alloc(ineed, ioff)) {
+    harena.release(hoff, hneed);
+    if (dneed) darena.release(doff, dneed);
     return CUCM_E_NOTREADY;
   }
   j.h_off = hoff;
   j.h_len = hneed;
   j.d_off = doff;
   j.d_len = dneed;
+  j.i_off = ioff;
+  j.i_len = ineed;
+  uint8_t *dib = (uint8_t *)dinput.p + ioff;
   j.slot = (int)free_slots.back();
   free_slots.pop_back();
   slot_job[j.slot] = id;
@@ -450,18 +471,29 @@ int Server::admit(uint64_t id, SJob &j) {
   if (j.gen == 0) j.gen = (++gen) & 0x7fffffffu;
   uint8_t *hb = hbase + hoff;
   if (q.in_len) std::memcpy(hb + in_off, q.in, q.in_len);
+  bool copy_ok = !q.in_len || hipMemcpyAsync(dib, hb + in_off, q.in_len, hipMemcpyHostToDevice,
+                                             cs) == hipSuccess;
   uint8_t *out = hb + out_off;
   j.out_off = hoff + out_off;
   uint32_t mflag = 0;
   if (model) {
     std::memcpy(hb + st_off, model->states.data() + ((size_t)1 << tb), states);
+    copy_ok = copy_ok && hipMemcpyAsync(dib + up256(q.in_len), hb + st_off, states,
+                                        hipMemcpyHostToDevice, cs) == hipSuccess;
     DevInitModel &d = models[j.slot];
-    d.states = harena_mem.dev<const uint8_t>(hb + st_off);
+    d.states = dib + up256(q.in_len);
     std::memcpy(d.statep, model->statep, sizeof d.statep);
     std::memcpy(d.w, model->w, sizeof d.w);
     order0_states(*model, d.st0);
     d.tb = tb;
     mflag = kTaskInitModel;
+  }
+  // the tasks read the device copy: it must be complete before they are published
+  if (copy_ok && hipStreamSynchronize(cs) != hipSuccess) copy_ok = false;
+  if (!copy_ok) {
+    (void)hipGetLastError();
+    release(j);
+    return CUCM_E_DEVICE;
   }
   j.t_start = clock_ns();
   if (q.op == CUCM_OP_COMPRESS) {
@@ -493,12 +525,12 @@ int Server::admit(uint64_t id, SJob &j) {
     t.model = (uint32_t)j.slot;
     t.job = (uint32_t)j.slot;
     if (q.op == CUCM_OP_COMPRESS) {
-      t.src = (uint64_t)(uintptr_t)harena_mem.dev<uint8_t>(hb + in_off + P + c * S);
+      t.src = (uint64_t)(uintptr_t)(dib + in_off + P + c * S);
       t.dst = (uint64_t)(uintptr_t)((uint8_t *)staging.p + doff + c * cap);
       t.aux = (uint32_t)cap;
       t.flags = (uint8_t)(((pad && c == C - 1) ? kTaskTailPad : 0) | mflag);
     } else {
-      t.src = (uint64_t)(uintptr_t)harena_mem.dev<uint8_t>(hb + in_off + pays[c].first);
+      t.src = (uint64_t)(uintptr_t)(dib + in_off + pays[c].first);
       t.aux = pays[c].second;
       t.dst = (uint64_t)(uintptr_t)((uint8_t *)staging.p + doff + c * S);
       t.flags = (uint8_t)(kTaskDecode | mflag);
