@@ -325,8 +325,9 @@ def main():
                     help="batch workload: requests of the serving mix measured beside it (0 = none)")
     ap.add_argument("--lanes", type=int, default=0, help="serve workload: concurrent lanes (0 = 3)")
     ap.add_argument("--replay-grid", default="",
-                    help="replay workload: D:waves:flags entries, comma separated (default: the "
-                         "decoder's residencies and the variants of cucm_probe_replay_decode)")
+                    help="replay workload: D:waves:flags[:chunk KiB] entries, comma separated "
+                         "(default: the decoder's residencies and the variants of "
+                         "cucm_probe_replay_decode, 128 KiB chunks)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the codec shards by chunk with no exchange on the "

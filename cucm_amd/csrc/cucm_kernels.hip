@@ -318,12 +318,18 @@ __global__ void __launch_bounds__(512) serve_kernel(ServeArgs s) {
       }
       stop = __builtin_amdgcn_readfirstlane(stop);
       if (stop || now - idle0 > s.idle_ticks) break;
-      // look often just after a task, then every ~25 us: hundreds of idle
-      // waves polling device memory faster slow the working waves' gathers
-      if (++spins < 64) {
+      // look often just after a task, then back off: every ~25 us, doubling
+      // every 16 looks up to ~0.8 ms.  Every idle wave's look reads the same
+      // few words, and hundreds of idle waves looking often slow the working
+      // waves' gathers (seven waves per CU: the last tasks of a saturated run
+      // took several times as long).  Any idle wave takes a new task, so the
+      // chip as a whole still looks every few microseconds.
+      ++spins;
+      if (spins < 64) {
         __builtin_amdgcn_s_sleep(32);
       } else {
-        for (int k = 0; k < 8; ++k) __builtin_amdgcn_s_sleep(127);
+        const uint32_t reps = 8u << min((spins - 64u) >> 4, 5u);
+        for (uint32_t k = 0; k < reps; ++k) __builtin_amdgcn_s_sleep(127);
       }
       continue;
     }

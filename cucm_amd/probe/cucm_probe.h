@@ -29,7 +29,8 @@ int cucm_probe_copy(void *dst, const void *src, uint64_t bytes, double *ms);
  * (path, order) gathers across lanes, one instruction per 9 paths; bit 3:
  * re-gather the unit's slots before its write-backs; bit 4: write each
  * round's first-path slots right after their gathers instead of at the unit
- * end.  *ms: kernel time. */
+ * end; bit 5: an agent-scope release and acquire fence after every chunk.
+ * *ms: kernel time. */
 int cucm_probe_replay_decode(const uint64_t *h_src, const uint32_t *h_len, uint32_t n,
                              uint32_t tb, uint32_t D, uint32_t U, uint32_t waves,
                              uint32_t flags, double *ms);

@@ -213,6 +213,13 @@ __global__ void __launch_bounds__(256) replay_decode_kernel(const ReplayChunk *c
       }
       __builtin_amdgcn_wave_barrier();
     }
+    if (zero_slot & 32u) {
+      // the fence pair a persistent kernel runs per task to publish its
+      // results and take the next task: an agent-scope release (L2 write-back
+      // on this chip) and an agent-scope acquire (L2 invalidate)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
   }
   if (acc == 0xdeadbeefu) atomicAdd(sink, 1u);
 }

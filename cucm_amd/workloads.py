@@ -509,32 +509,43 @@ def replay_workload(args):
     ctx = Context()
     d_in = ctx.alloc(total)
     d_in.upload(arena)
-    src, lens = [], []
-    for o, n in zip(offs, sizes):
-        o, n = int(o), int(n)
-        for c in range(0, n, 131072):
-            src.append(d_in.addr + o + c)
-            lens.append(min(131072, n - c))
-    srca = (ctypes.c_uint64 * len(src))(*src)
-    lena = (ctypes.c_uint32 * len(lens))(*lens)
+    chunkings = {}
+
+    def chunk_list(kib):
+        # the batch's files cut into chunks of at most kib KiB
+        if kib not in chunkings:
+            src, lens = [], []
+            for o, n in zip(offs, sizes):
+                o, n = int(o), int(n)
+                for c in range(0, n, kib * 1024):
+                    src.append(d_in.addr + o + c)
+                    lens.append(min(kib * 1024, n - c))
+            chunkings[kib] = ((ctypes.c_uint64 * len(src))(*src),
+                              (ctypes.c_uint32 * len(lens))(*lens), len(src))
+        return chunkings[kib]
+
     out = []
-    # (speculation bits, resident waves, flags: 1 zero the slot per chunk, 2 no write-backs,
-    # 4 pack each round's gathers into one instruction, 8 re-gather before the write-backs,
-    # 16 write right after each round's gathers); speculation bits 0: the encoder's stream
-    # (every gather of a unit at once, flags 1 and 2 only)
+    # (speculation bits, resident waves, flags[, chunk KiB (default 128)]); flags: 1 zero
+    # the slot per chunk, 2 no write-backs, 4 pack each round's gathers into one
+    # instruction, 8 re-gather before the write-backs, 16 write right after each round's
+    # gathers, 32 agent-scope release + acquire fences after every chunk; speculation
+    # bits 0: the encoder's stream (every gather of a unit at once, flags 1, 2 and 4 only)
     grid = [tuple(int(v) for v in e.split(":")) for e in args.replay_grid.split(",")] \
         if getattr(args, "replay_grid", "") else [
             (1, 1536, 1), (1, 3072, 1), (2, 1280, 1), (2, 1536, 1), (2, 2048, 1), (2, 3072, 1),
             (4, 1280, 1), (2, 1536, 0), (2, 1536, 3), (1, 3072, 3), (2, 1536, 5), (2, 1536, 9),
             (2, 1536, 17), (0, 768, 1), (0, 1536, 1), (0, 3072, 1), (0, 1536, 3),
         ]
-    for D, waves, flags in grid:
+    for entry in grid:
+        D, waves, flags = entry[:3]
+        kib = entry[3] if len(entry) > 3 else 128
+        srca, lena, nsrc = chunk_list(kib)
         ms = ctypes.c_double()
         if D == 0:
-            rc = probe_lib().cucm_probe_replay_encode(srca, lena, len(src), 18, 128, waves, flags,
+            rc = probe_lib().cucm_probe_replay_encode(srca, lena, nsrc, 18, 128, waves, flags,
                                                       ctypes.byref(ms))
         else:
-            rc = probe_lib().cucm_probe_replay_decode(srca, lena, len(src), 18, D, 128, waves,
+            rc = probe_lib().cucm_probe_replay_decode(srca, lena, nsrc, 18, D, 128, waves,
                                                       flags, ctypes.byref(ms))
         if rc:
             raise RuntimeError("replay probe failed: %d" % rc)
@@ -542,11 +553,12 @@ def replay_workload(args):
                     "spec_bits": D, "waves": waves, "zero_slot": bool(flags & 1),
                     "write_back": not (flags & 18), "packed_lanes": bool(flags & 4),
                     "regather_before_write": bool(flags & 8), "write_per_round": bool(flags & 16),
+                    "fences_per_chunk": bool(flags & 32), "chunk_kib": kib, "chunks": nsrc,
                     "ms": ms.value, "MBps": total / 1e3 / ms.value})
-        log("replay D=%d waves=%d flags=%d: %.0f ms, %.1f MB/s" % (D, waves, flags, ms.value,
-                                                                  out[-1]["MBps"]))
+        log("replay D=%d waves=%d flags=%d chunks=%d KiB: %.0f ms, %.1f MB/s"
+            % (D, waves, flags, kib, ms.value, out[-1]["MBps"]))
     return {"metric": "decoder gather-stream replay (no model arithmetic)", "bytes": total,
-            "chunks": len(src), "table_bits": 18, "unit_bits": 128, "results": out}
+            "table_bits": 18, "unit_bits": 128, "results": out}
 
 
 def copy_workload(args):
