@@ -324,6 +324,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=240,
                     help="batch workload: requests of the serving mix measured beside it (0 = none)")
     ap.add_argument("--lanes", type=int, default=0, help="serve workload: concurrent lanes (0 = 3)")
+    ap.add_argument("--phases", action="store_true",
+                    help="batch workload: one more decode in the profiling instantiation, its "
+                         "per-phase clocks per decoded bit in the line")
     ap.add_argument("--replay-grid", default="",
                     help="replay workload: D:waves:flags[:chunk KiB] entries, comma separated "
                          "(default: the decoder's residencies and the variants of "

@@ -168,6 +168,21 @@ def batch_workload(args):
     res["roofline"] = dict({"bound": "hbm", "kernel": kname, "achieved": achieved,
                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS},
                            **batch_traffic(kname, total, kms / 1e3))
+    if getattr(args, "phases", False):
+        # one more decode in the profiling instantiation (per-phase shader
+        # clocks of every wave, summed; a few % slower than the timed kernel):
+        # where a decoded bit's time goes
+        apply_launch(ctx, dec)
+        ctx.set_phase_profile(True)
+        ctx.decompress_many_device(cs, cl, outs, lens, spec)
+        ph, sp = ctx.phase_cycles(), ctx.stats()
+        ctx.set_phase_profile(False)
+        per_bit = {k: v / (8.0 * total) for k, v in ph.items() if v}
+        res["decoder_phases"] = {"kernel_ms": sp["codec_ms"], "waves": int(sp["waves"]),
+                                 "spec_bits": int(sp["spec_bits"]),
+                                 "wave_clocks_per_decoded_bit": per_bit}
+        log("decoder phases (%.1f ms): " % sp["codec_ms"]
+            + ", ".join("%s %.0f" % (k, v) for k, v in per_bit.items()))
     # plain path: one launch per file on a seeded sample of the same files
     plain_files = getattr(args, "plain_files", 40)
     if plain_files:
