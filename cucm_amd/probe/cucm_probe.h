@@ -38,7 +38,8 @@ int cucm_probe_replay_decode(const uint64_t *h_src, const uint32_t *h_len, uint3
  * 7 gathers of every bit at once, then the unit's 7 states per bit written
  * back and drained before the next unit.  flags bit 0: zero the slot per
  * chunk; bit 1: skip the write-backs; bit 2: read each state's whole 32-byte
- * sector and write the sector back whole (U <= 128). */
+ * sector and write the sector back whole (U <= 128); bit 3: do not wait for
+ * a unit's write-backs before the next unit's gathers. */
 int cucm_probe_replay_encode(const uint64_t *h_src, const uint32_t *h_len, uint32_t n,
                              uint32_t tb, uint32_t U, uint32_t waves, uint32_t flags,
                              double *ms);

@@ -336,7 +336,9 @@ __global__ void __launch_bounds__(256) replay_encode_kernel(const ReplayChunk *c
 #pragma unroll
             for (int k = 0; k < 7; ++k) slot[pos[j][k]] = (uint8_t)(v + k + dep);
       }
-      __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the unit's writes land first
+      // vmcnt(0): the unit's writes land first (flags bit 3: the next unit's
+      // gathers go out while they are in flight)
+      if (!(flags & 8u)) __builtin_amdgcn_s_waitcnt(0x0f70);
       __builtin_amdgcn_wave_barrier();
     }
   }

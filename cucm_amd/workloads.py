@@ -544,7 +544,8 @@ def replay_workload(args):
     # the slot per chunk, 2 no write-backs, 4 pack each round's gathers into one
     # instruction, 8 re-gather before the write-backs, 16 write right after each round's
     # gathers, 32 agent-scope release + acquire fences after every chunk; speculation
-    # bits 0: the encoder's stream (every gather of a unit at once, flags 1, 2 and 4 only)
+    # bits 0: the encoder's stream (every gather of a unit at once; flags 1, 2, 4, and 8: no
+    # wait for a unit's write-backs before the next unit's gathers)
     grid = [tuple(int(v) for v in e.split(":")) for e in args.replay_grid.split(",")] \
         if getattr(args, "replay_grid", "") else [
             (1, 1536, 1), (1, 3072, 1), (2, 1280, 1), (2, 1536, 1), (2, 2048, 1), (2, 3072, 1),
@@ -567,7 +568,9 @@ def replay_workload(args):
         out.append({"stream": "encoder" if D == 0 else "decoder",
                     "spec_bits": D, "waves": waves, "zero_slot": bool(flags & 1),
                     "write_back": not (flags & 18), "packed_lanes": bool(flags & 4),
-                    "regather_before_write": bool(flags & 8), "write_per_round": bool(flags & 16),
+                    "regather_before_write": bool(D and flags & 8),
+                    "no_drain_before_next_unit": bool(not D and flags & 8),
+                    "write_per_round": bool(flags & 16),
                     "fences_per_chunk": bool(flags & 32), "chunk_kib": kib, "chunks": nsrc,
                     "ms": ms.value, "MBps": total / 1e3 / ms.value})
         log("replay D=%d waves=%d flags=%d chunks=%d KiB: %.0f ms, %.1f MB/s"
