@@ -413,9 +413,7 @@ CUCM_DEV void init_model(const CodecArgs &a, const Task &tk, const WaveMem &M, G
 //      cleared for the next unit.
 // ym receives the unit's bits y as uniform masks for the statep walk (the
 // second part, the statep walk), which may run while other work proceeds.
-// kDrain: wait for the write-backs here; without it the caller must run
-// vm_drain() before the wave's next gathers.
-template <int NJ, bool kDrain = true>
+template <int NJ>
 CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_t *slot,
                             uint32_t tb, uint32_t Umax, uint32_t nvalid, uint32_t lane,
                             PhaseClock &clk, uint64_t ym[NJ]) {
@@ -521,7 +519,7 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
       const uint32_t i = rix_get<NJ>(M, k * Umax + r);
       M.tkey[(k - 1) * TS + i] = kEmpty;
     }
-  if constexpr (kDrain) vm_drain();
+  vm_drain();
   wave_sync();
   clk.mark(8);
 }
@@ -613,6 +611,35 @@ CUCM_DEV void update_states(const WaveMem &M, const LDS DevTables *T, GLB uint8_
     w_cur = w_nxt;                                                                        \
     w_nxt = (w_blk + 1 < w_nblk) ? *(const LDS uint64_t *)(w_sv + 8 * (w_blk + 1)) : w_cur; \
   } while (0)
+
+// y of unit bit r from the masks update_states returns.
+template <int NJ>
+CUCM_DEV uint32_t unit_y(const uint64_t ym[NJ], uint32_t r) {
+  uint64_t w = ym[0];
+#pragma unroll
+  for (int q = 1; q < NJ; ++q)
+    if ((r >> 6) == (uint32_t)q) w = ym[q];
+  return (uint32_t)(w >> (r & 63u)) & 1u;
+}
+
+// The whole statep walk of a unit on its own.
+template <int NJ>
+CUCM_DEV void statep_walk(const WaveMem &M, const LDS DevTables *T, uint32_t Umax, uint32_t nvalid,
+                          uint32_t lane, const uint64_t ym[NJ]) {
+  CUCM_WALK_STATE;
+  CUCM_WALK_INIT(M, T, Umax, nvalid, lane);
+  for (uint32_t blk = 0; blk < nvalid / 8; ++blk) {
+    uint64_t w = ym[0];
+#pragma unroll
+    for (int q = 1; q < NJ; ++q)
+      if ((blk >> 3) == (uint32_t)q) w = ym[q];
+    const uint32_t yb = (uint32_t)(w >> (8 * (blk & 7u))) & 255u;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) CUCM_WALK_STEP(t, (yb >> t) & 1u);
+    CUCM_WALK_NEXT_BLOCK();
+  }
+  wave_sync();
+}
 
 // ---------------------------------------------------------------------------
 // Encoder
@@ -1004,9 +1031,8 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
   uint64_t hist = 0;     // the 8 bytes before the current one, latest lowest
   uint32_t partial = 0;  // decided bits of the current byte
   const uint64_t vbits = 8 * L;
-  // the first round's gathers of every later unit are issued halfway through
-  // the previous unit's statep walk (they read only HBM states, written back
-  // and drained before they go out)
+  // the first round's gathers of every unit are issued before the previous
+  // unit's statep walk (they read only the states the write-back stored)
   uint32_t g[NM][8];
   if (vbits) paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, 0, lane, g);
   for (uint64_t base = 0; base < vbits; base += U) {
@@ -1101,34 +1127,11 @@ CUCM_DEV void decode_task(const CodecArgs &a, const Task &tk, uint32_t t,
       clk.mark(11);
     }
     uint64_t ym[NJ];
-    update_states<NJ, false>(M, T, slot, tb, Umax, nbits, lane, clk, ym);
+    update_states<NJ>(M, T, slot, tb, Umax, nbits, lane, clk, ym);
     const uint32_t nb = nbits >> 3;
     for (uint32_t i = lane; i < nb; i += 64) dst[(base >> 3) + i] = M.ub[8 + i];
-    // The statep walk (LDS only).  Its first half runs while the unit's
-    // write-backs land; then the next unit's first gathers go out (they may
-    // read the slots just written, so after the drain) and fly during the
-    // second half.
-    {
-      CUCM_WALK_STATE;
-      CUCM_WALK_INIT(M, T, Umax, nbits, lane);
-      const uint32_t half = nb >> 1;
-      for (uint32_t blk = 0; blk < nb; ++blk) {
-        if (blk == half) {
-          vm_drain();
-          if (base + U < vbits)
-            paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, 0, lane, g);
-        }
-        uint64_t w = ym[0];
-#pragma unroll
-        for (int q = 1; q < NJ; ++q)
-          if ((blk >> 3) == (uint32_t)q) w = ym[q];
-        const uint32_t yb = (uint32_t)(w >> (8 * (blk & 7u))) & 255u;
-#pragma unroll
-        for (int t = 0; t < 8; ++t) CUCM_WALK_STEP(t, (yb >> t) & 1u);
-        CUCM_WALK_NEXT_BLOCK();
-      }
-      wave_sync();
-    }
+    if (base + U < vbits) paths_issue<NM, P>(M, slot, tb, mask, 0, 0, hist, 0, lane, g);
+    statep_walk<NJ>(M, T, Umax, nbits, lane, ym);
     clk.mark(7);
   }
   if (lane == 0) a.result[t] = ip;
