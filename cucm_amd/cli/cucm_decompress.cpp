@@ -1,9 +1,11 @@
 // cucm_decompress -- drop-in for the reference's decompressor command line
 // (cucm_decompress_v2.cu:1250-1313): -i input, -o output, -a
-// maxassumed_length (default 3: the reference decides maxassumed_length + 1
-// bits per speculative round; here the width is the largest of 1, 2, 4, 8
-// not above that, and it never changes the output).  Batch records stream
-// through the C ABI, about 256 MiB of output at a time.  Extra: -d device.
+// maxassumed_length (the reference decides maxassumed_length + 1 bits per
+// speculative round, 3 + 1 by default; here -a m selects the largest width
+// of 1, 2, 4, 8 not above m + 1, and without -a the library chooses the
+// width with the launch shape -- neither ever changes the output).  Batch
+// records stream through the C ABI, about 256 MiB of output at a time.
+// Extra: -d device.
 #include <chrono>
 #include <cstring>
 #include <iostream>
@@ -16,13 +18,14 @@ static void print_help(const char *prog) {
   std::cout << "Usage: " << prog << " -i <input_file> -o <output_file> [-a <maxassumed_length>]\n";
   std::cout << "  -i <input_file>        Input compressed file\n";
   std::cout << "  -o <output_file>       Output decompressed file\n";
-  std::cout << "  -a <N>                 Set maxassumed_length (default: 3)\n";
+  std::cout << "  -a <N>                 Set maxassumed_length (default: the library's automatic\n"
+               "                         speculation width; the output is the same for every N)\n";
   std::cout << "  -d <N>                 GPU to use (default: 0)\n";
 }
 
 int main(int argc, char *argv[]) {
   const char *input_file = nullptr, *output_file = nullptr;
-  int maxassumed_length = 3, device = 0;
+  int maxassumed_length = -1, device = 0;  // -1: automatic width
   for (int i = 1; i < argc; ++i) {
     if (std::strcmp(argv[i], "-i") == 0 && i + 1 < argc) {
       input_file = argv[++i];
@@ -52,7 +55,8 @@ int main(int argc, char *argv[]) {
     return 1;
   }
   const uint32_t want = (uint32_t)std::max(1, maxassumed_length + 1);
-  const uint32_t spec = want >= 8 ? 8 : want >= 4 ? 4 : want >= 2 ? 2 : 1;
+  const uint32_t spec =
+      maxassumed_length < 0 ? 0 : want >= 8 ? 8 : want >= 4 ? 4 : want >= 2 ? 2 : 1;
   const auto t0 = std::chrono::high_resolution_clock::now();
   std::vector<uint8_t> head(21);
   uint64_t head_len = 0;
@@ -127,7 +131,12 @@ int main(int argc, char *argv[]) {
   std::fclose(out);
   const std::chrono::duration<double> el = std::chrono::high_resolution_clock::now() - t0;
   std::cout << "Decompression finished: " << input_file << " -> " << output_file << " (" << N
-            << " bytes, maxassumed_length=" << maxassumed_length << ")\n";
+            << " bytes, maxassumed_length=";
+  if (maxassumed_length < 0)
+    std::cout << "automatic";
+  else
+    std::cout << maxassumed_length;
+  std::cout << ")\n";
   std::cout << "Total time: " << el.count() << " sec" << std::endl;
   return 0;
 }

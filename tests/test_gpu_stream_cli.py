@@ -97,8 +97,8 @@ def test_cli_round_trip_identical_to_library(ctx, tmp_path):
                  "-p", "3000"])
     assert "Compression ratio:" in out
     assert enc.read_bytes() == ctx.compress(data, make_params(16384, 128, 4, 3000, 18))
-    for a in ("0", "1", "3", "7"):
-        _tool([DECOMP, "-i", str(enc), "-o", str(dec), "-a", a])
+    for a in (None, "0", "1", "3", "7"):
+        _tool([DECOMP, "-i", str(enc), "-o", str(dec)] + (["-a", a] if a else []))
         assert dec.read_bytes() == data
 
 
