@@ -3,7 +3,8 @@
 Reference interface (host C++): compress_file(ifstream, ofstream, n, &out_n,
 chunksize, unitlength, prelearn_size=0, batch_num=100)
 (cucm_compress_v2.cu:1151-1154); decompress(ifstream, n, ofstream, out_n,
-maxassumed_length=3) (cucm_decompress_v2.cu:1185); compress_batch /
+maxassumed_length=3) (cucm_decompress_v2.cu:1185; here maxassumed_length
+defaults to an automatic width); compress_batch /
 decompressbatch (cucm_compress_v2.cu:990-992, cucm_decompress_v2.cu:898-903).
 Every call goes through libcucm.so's C ABI; there is no CPU fallback.
 """
@@ -435,12 +436,17 @@ def compress_file(infile, outfile, chunksize=MiB, unitlength=128, prelearn_size=
     return len(blob)
 
 
-def decompress_file(infile, outfile, maxassumed_length=3):
+def decompress_file(infile, outfile, maxassumed_length=None):
     """Mirrors decompress (cucm_decompress_v2.cu:1185).  The reference decides
-    maxassumed_length+1 bits per speculative round; here the round width is
-    rounded to 1, 2, 4 or 8 bits (it never changes the output)."""
-    width = maxassumed_length + 1
-    spec = max(b for b in (1, 2, 4, 8) if b <= max(width, 1))
+    maxassumed_length+1 bits per speculative round (3 + 1 by default); here
+    that width is rounded down to 1, 2, 4 or 8 bits, and None (the default)
+    lets the library choose it with the launch shape.  The width never
+    changes the output."""
+    if maxassumed_length is None:
+        spec = 0
+    else:
+        width = maxassumed_length + 1
+        spec = max(b for b in (1, 2, 4, 8) if b <= max(width, 1))
     with open(infile, "rb") as f:
         blob = f.read()
     data = decompress(blob, spec)

@@ -212,6 +212,23 @@ def test_one_shot_api(oracle):
     assert cucm_amd.decompress(blob) == data
 
 
+def test_file_api(oracle, tmp_path):
+    """compress_file / decompress_file (the reference's file-level calls):
+    the container equals the oracle's, and every maxassumed_length, and the
+    automatic width, decode it."""
+    data = datagen.markov2(60000, 27).tobytes()
+    src, enc, dec = tmp_path / "in", tmp_path / "in.cucm", tmp_path / "out"
+    src.write_bytes(data)
+    n = cucm_amd.compress_file(str(src), str(enc), chunksize=8192, unitlength=128,
+                               prelearn_size=2000, batch_num=3, table_bits=16)
+    assert enc.read_bytes() == oracle.compress(data, 8192, 128, 3, 2000, 16)
+    assert n == enc.stat().st_size
+    for m in (None, 0, 1, 3, 7):
+        dec.unlink(missing_ok=True)
+        assert cucm_amd.decompress_file(str(enc), str(dec), maxassumed_length=m) == len(data)
+        assert dec.read_bytes() == data, m
+
+
 def test_small_budget_is_refused_cleanly():
     with cucm_amd.Context(0, hbm_budget=1 << 20) as c:
         with pytest.raises(cucm_amd.CucmError) as e:
