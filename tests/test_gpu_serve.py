@@ -1,6 +1,6 @@
 """Serving loop (cucm_serve_start / cucm_submit / cucm_poll): requests of
-mixed kinds, sizes and parameters (frozen-model ones among them) submitted
-together share the serving kernel's running launch (or, with static lanes,
+mixed kinds, sizes and parameters (prelearned- and frozen-model ones among
+them) submitted together share the serving kernel's running launch (or, with static lanes,
 shared batches), requests that arrive mid-launch are admitted into it, and
 every result equals the same request run alone -- bit-exact, since the
 codec's output depends only on the request."""
@@ -19,9 +19,11 @@ def _requests():
     for i in range(40):
         n = int(rng.choice([0, 1, 7, 300, 5000, 40000, 150000, 400000]))
         data = datagen.markov2(n, 300 + i).tobytes() if n else b""
-        kind = i % 5
+        kind = i % 6
         if kind == 0:
             p = make_params(16384, 128, 100, 0, 18)
+        elif kind == 5:  # prelearned model, taken by the serving kernel (U <= 128, 2^18)
+            p = make_params(16384, 128, 5, min(n, 4000), 18)
         elif kind == 1:
             p = make_params(32768, 64, 3, 0, 16)
         elif kind == 2:
