@@ -64,6 +64,23 @@ def test_small_config(ctx, oracle):
     assert ctx.decompress(got) == data
 
 
+@pytest.mark.parametrize("P", [0, 1 << 20], ids=["P0", "P1M"])
+def test_reference_default_configuration(ctx, oracle, P):
+    """The reference command line's defaults (cucm_compress_v2.cu:1288-1366)
+    with its hard-coded 2^25 slots per order (statemap_size,
+    cucm_compress_v2.cu:553): S = 1 MiB, U = 128, batch 100, with and without
+    a 1 MiB prelearn segment (-p 1M).  3 MiB + 4321 bytes of the mixed
+    stream, so the last chunk is ragged and tail-padded; the container is
+    the oracle's byte for byte and decodes at the automatic and a fixed
+    speculation width."""
+    data = _data("mixed", 3 * (1 << 20) + 4321, 91)
+    ref = oracle.compress(data, 1 << 20, 128, 100, P, 25)
+    got = ctx.compress(data, make_params(1 << 20, 128, 100, P, 25))
+    assert got == ref
+    assert ctx.decompress(got) == data
+    assert ctx.decompress(got, 2) == data
+
+
 def test_empty_input(ctx, oracle):
     ref = oracle.compress(b"", 4096, 128, 100, 0, 18)
     got = ctx.compress(b"", make_params(4096, 128, 100, 0, 18))
