@@ -95,10 +95,11 @@ def _run(ctx, ins, lens, params, cs, caps, outs, direction, launch, cl=None):
 
 
 # max_waves caps the resident waves (teams: 4 per chunk): on the 10,000-file
-# batch the encoder's teams run faster at 2.5 per CU than at the 3 LDS allows
-# (545.8 against 516.4 MB/s), since the memory system serves fewer chunks in
-# flight better
-RESIDENCY_GRID = (2048, 2560)
+# batch the encoder's teams run fastest at 2.25 per CU, not at the 3 LDS
+# allows (576 teams 3,650 ms, 640 teams 3,770 ms, 512 teams 3,805 ms, 768
+# teams ~3,980 ms, measured on one box), since the memory system serves
+# fewer chunks in flight better
+RESIDENCY_GRID = (2048, 2304, 2560)
 ENCODER_LAUNCH = [{"team": True, "waves_per_block": 0}] + \
     [{"team": True, "waves_per_block": 0, "max_waves": m} for m in RESIDENCY_GRID] + \
     [{"team": False, "waves_per_block": w} for w in WPB_GRID]
