@@ -247,14 +247,17 @@ Shape pick_shape(const cucm_ctx *c, uint64_t ntasks, uint32_t Umax, uint32_t spe
       wmax = w;
       break;
     }
+  // resident capacity for the workgroup size the launch uses (a fixed
+  // waves_per_block packs differently into a CU's LDS than wmax does)
+  const uint32_t wfit = c->waves_per_block ? std::min<uint32_t>(c->waves_per_block, wmax) : wmax;
   uint64_t waves = ntasks;
-  const uint64_t per_cu_blocks = std::max<uint64_t>(1, c->lds_cu / codec_lds_bytes(Umax, spec, dec, wmax));
-  waves = std::min<uint64_t>(waves, (uint64_t)c->cus * per_cu_blocks * wmax);
+  const uint64_t per_cu_blocks = std::max<uint64_t>(1, c->lds_cu / codec_lds_bytes(Umax, spec, dec, wfit));
+  waves = std::min<uint64_t>(waves, (uint64_t)c->cus * per_cu_blocks * wfit);
   if (c->max_waves) waves = std::min<uint64_t>(waves, c->max_waves);
   if (slot_bytes) waves = std::min<uint64_t>(waves, mem_left / slot_bytes);
   if (waves == 0) return s;
   uint32_t w = (uint32_t)std::min<uint64_t>(wmax, (waves + c->cus - 1) / c->cus);
-  if (c->waves_per_block) w = std::min<uint32_t>(c->waves_per_block, wmax);
+  if (c->waves_per_block) w = wfit;
   s.wpb = std::max<uint32_t>(w, 1);
   s.blocks = (uint32_t)((waves + s.wpb - 1) / s.wpb);
   if ((uint64_t)s.blocks * s.wpb > waves && s.blocks > 1) {
@@ -271,12 +274,13 @@ Shape frozen_shape(const cucm_ctx *c, uint64_t ntasks) {
   Shape s;
   uint32_t wmax = 8;
   while (wmax > 1 && frozen_lds_bytes(wmax) > c->lds_block) --wmax;
-  const uint64_t per_cu = std::max<uint64_t>(1, c->lds_cu / frozen_lds_bytes(wmax));
-  uint64_t waves = std::min<uint64_t>(ntasks, (uint64_t)c->cus * per_cu * wmax);
+  const uint32_t wfit = c->waves_per_block ? std::min<uint32_t>(c->waves_per_block, wmax) : wmax;
+  const uint64_t per_cu = std::max<uint64_t>(1, c->lds_cu / frozen_lds_bytes(wfit));
+  uint64_t waves = std::min<uint64_t>(ntasks, (uint64_t)c->cus * per_cu * wfit);
   if (c->max_waves) waves = std::min<uint64_t>(waves, c->max_waves);
   if (waves == 0) return s;
   uint32_t w = (uint32_t)std::min<uint64_t>(wmax, (waves + c->cus - 1) / c->cus);
-  if (c->waves_per_block) w = std::min<uint32_t>(c->waves_per_block, wmax);
+  if (c->waves_per_block) w = wfit;
   s.wpb = std::max<uint32_t>(w, 1);
   s.blocks = (uint32_t)((waves + s.wpb - 1) / s.wpb);
   return s;

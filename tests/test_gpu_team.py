@@ -143,3 +143,30 @@ def test_auto_speculation_width_follows_residency(tctx, oracle):
             assert (st["spec_bits"], min(st["teams"], 1)) == want
         finally:
             tctx.set_team(0)
+
+
+def test_fixed_waves_per_block_plans_only_resident_workgroups(tctx, oracle):
+    """cucm_ctx_set_waves_per_block: the launch is sized by how many
+    workgroups of that size the CUs' LDS holds at once, so every planned
+    workgroup is resident (no model slot is reserved for a workgroup that
+    could only start once the queue is empty).  One wave per chunk, 2-bit
+    rounds, 2,400 chunks: seven waves per workgroup fit one workgroup per
+    CU, and so do six, so six-wave launches plan no more workgroups than
+    seven-wave ones."""
+    many = _data("markov2", 2400 * 256, 13)
+    pm = make_params(256, 128, 100, 0, 10)
+    bm = tctx.compress(many, pm)
+    assert bm == oracle.compress(many, 256, 128, 100, 0, 10)
+    tctx.set_team(1)
+    try:
+        shapes = {}
+        for wpb in (7, 6):
+            tctx.set_waves_per_block(wpb)
+            assert tctx.decompress(bm, 2) == many
+            st = tctx.stats()
+            assert st["waves"] % wpb == 0
+            shapes[wpb] = st["waves"] // wpb
+        assert shapes[6] <= shapes[7], shapes
+    finally:
+        tctx.set_waves_per_block(0)
+        tctx.set_team(0)
