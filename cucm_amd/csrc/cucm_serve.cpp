@@ -36,6 +36,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <thread>
 #include <unordered_map>
 
@@ -133,8 +134,25 @@ struct Shared {
   }
 };
 
+// What a request's admission computes from the request alone: its
+// parameters checked; for compression the container header and prelearn
+// segment, for decompression the header, the batch records' chunk payloads
+// and the prelearned model.  Made once per request by the scheduler thread
+// outside the server lock (the prelearn coder is serial), and kept while the
+// request waits for a job slot, ring room or arena memory.
+struct AdmitPrep {
+  int rc = CUCM_OK;
+  std::vector<uint8_t> head_bytes;                   // compress: header + prelearn segment
+  std::unique_ptr<HostModel> model;                  // prelearned model (P > 0)
+  uint64_t S = 0, C = 0, P = 0, N = 0;
+  uint32_t U = 0, tb = 0, batch = 0;
+  std::vector<std::pair<uint64_t, uint32_t>> pays;  // decompress: (offset, length) per chunk
+  std::vector<uint8_t> pre;                          // decompress: the prelearn bytes
+};
+
 struct SJob {
   cucm_job job{};
+  std::shared_ptr<AdmitPrep> prep;  // serving kernel: admission inputs, once made
   uint64_t t_submit = 0, t_start = 0, t_done = 0;
   int32_t status = 0;
   uint32_t batch = 0;
@@ -343,93 +361,118 @@ void Server::publish(SJob &j, uint32_t first, uint32_t n) {
 // Admission of one request: CUCM_E_NOTREADY when a slot, ring room or arena
 // memory is not free yet.  On success the request's tasks are published (or,
 // with no chunks, it is complete).
-int Server::admit(uint64_t id, SJob &j) {
-  if (free_slots.empty() || inflight_tasks >= kRing) return CUCM_E_NOTREADY;
-  const cucm_job &q = j.job;
-  std::vector<uint8_t> head_bytes;  // compress: header + prelearn segment
-  HostModel *model = nullptr;
-  std::unique_ptr<HostModel> mhold;
-  uint64_t S = 0, C = 0, P = 0, N = 0, seg = 0;
-  uint32_t U = 0, tb = 0, batch = 0;
-  std::vector<std::pair<uint64_t, uint32_t>> pays;  // decompress: (offset, length) per chunk
-  std::vector<uint8_t> pre;                          // decompress: the prelearn bytes
+// The request-only part of admission (see AdmitPrep).
+static void prepare_request(const cucm_job &q, AdmitPrep &a) {
+  uint64_t seg = 0;
   if (q.op == CUCM_OP_COMPRESS) {
     const cucm_params &p = q.params;
-    const int rc = validate_params(p, q.in_len);
-    if (rc) return rc;
-    S = p.chunk_size;
-    P = p.prelearn_size;
-    U = p.unit_bits;
-    tb = p.table_bits;
-    batch = p.batch_chunks;
-    C = (q.in_len - P + S - 1) / S;
+    if ((a.rc = validate_params(p, q.in_len))) return;
+    a.S = p.chunk_size;
+    a.P = p.prelearn_size;
+    a.U = p.unit_bits;
+    a.tb = p.table_bits;
+    a.batch = p.batch_chunks;
+    a.C = (q.in_len - a.P + a.S - 1) / a.S;
     uint32_t Pc = 0, flag = 0;
     std::vector<uint8_t> segb;
-    if (P) {
-      mhold.reset(new HostModel((int)tb));
-      model = mhold.get();
+    if (a.P) {
+      a.model.reset(new HostModel((int)a.tb));
       std::vector<uint8_t> coded;
-      prelearn_encode(q.in, P, *model, coded);
+      prelearn_encode(q.in, a.P, *a.model, coded);
       Pc = (uint32_t)coded.size();
-      flag = P <= Pc ? 0u : 1u;
+      flag = a.P <= Pc ? 0u : 1u;
       if (flag)
         segb = coded;
       else
-        segb.assign(q.in, q.in + P);
+        segb.assign(q.in, q.in + a.P);
     }
-    head_bytes.resize(21 + segb.size());
-    write_header(head_bytes.data(), q.in_len, p, Pc, flag);
-    if (!segb.empty()) std::memcpy(head_bytes.data() + 21, segb.data(), segb.size());
-  } else {
-    if (q.in_len < 21) return CUCM_E_CORRUPT;
-    const uint8_t *h = q.in;
-    N = be32(h);
-    batch = ((uint32_t)h[4] << 8) | h[5];
-    const uint32_t uf = ((uint32_t)h[6] << 8) | h[7];
-    S = be32(h + 8);
-    const uint64_t Pc = be32(h + 12);
-    P = be32(h + 16);
-    const uint32_t flag = h[20] & 1u;
-    U = uf & ~31u;
-    tb = (uf & 31u) ? (uf & 31u) : 25u;
-    seg = flag ? Pc : P;
-    if (P > N || h[20] > (1u | kHeaderFrozen) || tb < 8 || 21 + seg > q.in_len)
-      return CUCM_E_CORRUPT;
-    const uint64_t body = N - P;
-    if (body > 0 && (S == 0 || U == 0 || batch == 0)) return CUCM_E_CORRUPT;
-    if (N > q.out_cap) return CUCM_E_SMALL;
-    C = body ? (body + S - 1) / S : 0;
-    if (C > (q.in_len - 21 - seg) / 5) return CUCM_E_CORRUPT;
-    // batch records, as build_decode_tasks_kernel walks them
-    uint64_t pos = 21 + seg;
-    for (uint64_t c0 = 0; c0 < C; c0 += batch) {
-      const uint64_t Cb = std::min<uint64_t>(batch, C - c0);
-      if (pos + 4 > q.in_len) return CUCM_E_CORRUPT;
-      const uint64_t blen = le32(q.in + pos);
-      if (pos + 4 + blen > q.in_len || blen < 5 * Cb) return CUCM_E_CORRUPT;
-      const uint8_t *blob = q.in + pos + 4;
-      uint64_t prev = 4 * Cb - 1;
-      for (uint64_t c = 0; c < Cb; ++c) {
-        const uint64_t E = be32(blob + 4 * c);
-        if (E <= prev || E >= blen) return CUCM_E_CORRUPT;
-        pays.push_back({pos + 4 + prev + 1, (uint32_t)(E - prev)});
-        prev = E;
-      }
-      if (prev != blen - 1) return CUCM_E_CORRUPT;
-      pos += 4 + blen;
+    a.head_bytes.resize(21 + segb.size());
+    write_header(a.head_bytes.data(), q.in_len, p, Pc, flag);
+    if (!segb.empty()) std::memcpy(a.head_bytes.data() + 21, segb.data(), segb.size());
+    return;
+  }
+  if (q.in_len < 21) {
+    a.rc = CUCM_E_CORRUPT;
+    return;
+  }
+  const uint8_t *h = q.in;
+  a.N = be32(h);
+  a.batch = ((uint32_t)h[4] << 8) | h[5];
+  const uint32_t uf = ((uint32_t)h[6] << 8) | h[7];
+  a.S = be32(h + 8);
+  const uint64_t Pc = be32(h + 12);
+  a.P = be32(h + 16);
+  const uint32_t flag = h[20] & 1u;
+  a.U = uf & ~31u;
+  a.tb = (uf & 31u) ? (uf & 31u) : 25u;
+  seg = flag ? Pc : a.P;
+  if (a.P > a.N || h[20] > (1u | kHeaderFrozen) || a.tb < 8 || 21 + seg > q.in_len) {
+    a.rc = CUCM_E_CORRUPT;
+    return;
+  }
+  const uint64_t body = a.N - a.P;
+  if (body > 0 && (a.S == 0 || a.U == 0 || a.batch == 0)) {
+    a.rc = CUCM_E_CORRUPT;
+    return;
+  }
+  if (a.N > q.out_cap) {
+    a.rc = CUCM_E_SMALL;
+    return;
+  }
+  a.C = body ? (body + a.S - 1) / a.S : 0;
+  if (a.C > (q.in_len - 21 - seg) / 5) {
+    a.rc = CUCM_E_CORRUPT;
+    return;
+  }
+  // batch records, as build_decode_tasks_kernel walks them
+  uint64_t pos = 21 + seg;
+  for (uint64_t c0 = 0; c0 < a.C; c0 += a.batch) {
+    const uint64_t Cb = std::min<uint64_t>(a.batch, a.C - c0);
+    const uint64_t blen = pos + 4 <= q.in_len ? le32(q.in + pos) : 0;
+    if (pos + 4 > q.in_len || pos + 4 + blen > q.in_len || blen < 5 * Cb) {
+      a.rc = CUCM_E_CORRUPT;
+      return;
     }
-    if (P) {
-      mhold.reset(new HostModel((int)tb));
-      model = mhold.get();
-      pre.resize(P);
-      if (flag) {
-        prelearn_decode(q.in + 21, seg, P, *model, pre.data());
-      } else {
-        std::memcpy(pre.data(), q.in + 21, P);
-        prelearn_train(pre.data(), P, *model);
+    const uint8_t *blob = q.in + pos + 4;
+    uint64_t prev = 4 * Cb - 1;
+    for (uint64_t c = 0; c < Cb; ++c) {
+      const uint64_t E = be32(blob + 4 * c);
+      if (E <= prev || E >= blen) {
+        a.rc = CUCM_E_CORRUPT;
+        return;
       }
+      a.pays.push_back({pos + 4 + prev + 1, (uint32_t)(E - prev)});
+      prev = E;
+    }
+    if (prev != blen - 1) {
+      a.rc = CUCM_E_CORRUPT;
+      return;
+    }
+    pos += 4 + blen;
+  }
+  if (a.P) {
+    a.model.reset(new HostModel((int)a.tb));
+    a.pre.resize(a.P);
+    if (flag) {
+      prelearn_decode(q.in + 21, seg, a.P, *a.model, a.pre.data());
+    } else {
+      std::memcpy(a.pre.data(), q.in + 21, a.P);
+      prelearn_train(a.pre.data(), a.P, *a.model);
     }
   }
+}
+
+int Server::admit(uint64_t id, SJob &j) {
+  if (free_slots.empty() || inflight_tasks >= kRing) return CUCM_E_NOTREADY;
+  const cucm_job &q = j.job;
+  const AdmitPrep &a = *j.prep;
+  if (a.rc) return a.rc;
+  const uint64_t S = a.S, C = a.C, P = a.P, N = a.N;
+  const uint32_t U = a.U, tb = a.tb, batch = a.batch;
+  const HostModel *model = a.model.get();
+  const std::vector<uint8_t> &head_bytes = a.head_bytes;
+  const std::vector<std::pair<uint64_t, uint32_t>> &pays = a.pays;
+  const std::vector<uint8_t> &pre = a.pre;
   if (C > kRing) return CUCM_E_UNSUPPORTED;  // more chunks than the ring: a fallback lane
   if (inflight_tasks + C > kRing) return CUCM_E_NOTREADY;
   // memory: input | output | model states in the shared arena; staging on the device
@@ -628,16 +671,29 @@ void Server::sched_loop() {
         progress = true;
         continue;
       }
-      const int rc = admit(id, j);
+      if (!j.prep) {
+        // the request-only work (a serial prelearn coder among it) runs
+        // once, without the lock, so submit and poll are not held up; only
+        // this thread admits, so the request stays at the queue's front
+        const cucm_job q = j.job;
+        auto prep = std::make_shared<AdmitPrep>();
+        lk.unlock();
+        prepare_request(q, *prep);
+        lk.lock();
+        jobs[id].prep = std::move(prep);
+      }
+      SJob &ja = jobs[id];
+      const int rc = admit(id, ja);
       if (rc == CUCM_E_NOTREADY) break;
+      ja.prep.reset();
       pending.pop_front();
       progress = true;
       if (rc == CUCM_E_UNSUPPORTED) {
         fb_pending.push_back(id);
         cv_work.notify_one();
       } else if (rc) {
-        j.t_start = clock_ns();
-        complete(j, rc, 0);
+        ja.t_start = clock_ns();
+        complete(ja, rc, 0);
       }
     }
     // completions
