@@ -170,3 +170,25 @@ def test_fixed_waves_per_block_plans_only_resident_workgroups(tctx, oracle):
     finally:
         tctx.set_waves_per_block(0)
         tctx.set_team(0)
+
+
+def test_many_pass_encoder_teams_are_capped(tctx, oracle):
+    """An encoder launch with four or more full passes of teams keeps at most
+    9 teams per 4 CUs resident by default (fewer than LDS allows: measured
+    faster); an explicit resident-wave cap replaces that default.  The
+    output never changes."""
+    many = _data("markov2", 4000 * 256, 14)
+    pm = make_params(256, 128, 100, 0, 10)
+    ref = oracle.compress(many, 256, 128, 100, 0, 10)
+    tctx.set_team(2)
+    try:
+        assert tctx.compress(many, pm) == ref
+        capped = tctx.stats()["teams"]
+        tctx.set_max_waves(4 * 4000)
+        assert tctx.compress(many, pm) == ref
+        full = tctx.stats()["teams"]
+    finally:
+        tctx.set_max_waves(0)
+        tctx.set_team(0)
+    assert 0 < capped < full and capped * 4 <= full * 3, (capped, full)
+    assert tctx.decompress(ref) == many
